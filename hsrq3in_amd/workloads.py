@@ -1,0 +1,143 @@
+"""Synthetic problem generators for robust inverse iteration.
+
+Every generator is seeded and deterministic (numpy PCG64), so a matrix built
+here and one rebuilt on another host from the same seed are bit-identical;
+that is what lets the (expensive, O(n^3)) shift lists be computed once and
+stored as small fixture files.
+
+Matrices are returned column-major (Fortran order), the layout of the C
+entry point ``tiled_inverse_iteration`` (reference ``inverse_iteration.h:19-22``).
+
+Generators
+----------
+* ``gaussian_hessenberg(n, seed)`` -- H[i, j] ~ N(0, 1) for i <= j + 1, zero
+  below the subdiagonal (the workload of the plain and batched benchmarks).
+* ``graded(H0, kmax, seed)`` -- H = D H0 D^-1, D = diag(2^k), k uniform in
+  [-kmax, kmax].  Power-of-two scaling is exact in FP64, so the spectrum is
+  that of H0 bit for bit and the eigenvector is D x0.
+* ``constructed_spectrum(lam, seed)`` -- an unreduced upper Hessenberg matrix
+  with a prescribed real spectrum, built in O(n^2) as H = U B U^-1 with B
+  lower bidiagonal (diagonal = lam) and U upper bidiagonal.  An upper
+  triangular similarity keeps Hessenberg form, and B is triangular, so the
+  eigenvalues are lam up to the rounding of the construction.  Used where the
+  O(n^3) eigenvalue computation of a Gaussian matrix is too costly (eigenvalue
+  clusters, n >= 16000).
+* ``real_shifts(H, m)`` -- the first m real eigenvalues (LAPACK dgeev via
+  numpy: balancing + dhseqr) in returned order, perturbed by relative 1e-12.
+"""
+from __future__ import annotations
+
+import os
+import numpy as np
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "tests", "golden")
+
+SHIFT_PERTURBATION = 1e-12
+
+
+def gaussian_hessenberg(n: int, seed: int) -> np.ndarray:
+    """Upper Hessenberg, N(0,1) on and above the subdiagonal, column-major."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    a = np.triu(a, -1)
+    return np.asfortranarray(a)
+
+
+def grading_exponents(n: int, kmax: int, seed: int, frac: float = 1.0) -> np.ndarray:
+    """Integer exponents k_i in [-kmax, kmax]; a fraction ``1-frac`` of them 0."""
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    k = rng.integers(-kmax, kmax + 1, size=n)
+    if frac < 1.0:
+        keep = rng.random(n) < frac
+        k = np.where(keep, k, 0)
+    return k.astype(np.int64)
+
+
+def graded(h0: np.ndarray, kmax: int, seed: int, frac: float = 1.0) -> np.ndarray:
+    """H = D H0 D^-1 with D = diag(2^k): exact, same eigenvalues as H0."""
+    n = h0.shape[0]
+    k = grading_exponents(n, kmax, seed, frac)
+    h = np.ldexp(np.asarray(h0), (k[:, None] - k[None, :]).astype(np.int32))
+    return np.asfortranarray(h)
+
+
+def constructed_spectrum(lam: np.ndarray, seed: int, sub: float = 0.15,
+                         ratio: float = 0.9) -> np.ndarray:
+    """Unreduced upper Hessenberg H with eigenvalues ``lam`` (O(n^2) build).
+
+    B = diag(lam) + subdiag(nu) (lower bidiagonal, so eig(B) = lam exactly);
+    U = diag(d) + superdiag(u), |u_j / d_{j+1}| <= ratio < 1 (cond(U) modest).
+    H = U B U^-1 is computed column by column from  X U = U B.
+    """
+    lam = np.asarray(lam, dtype=np.float64)
+    n = lam.size
+    rng = np.random.default_rng(seed)
+    scale = max(1.0, float(np.max(np.abs(lam)))) if n else 1.0
+    nu = sub * scale * rng.uniform(0.5, 1.0, size=max(n - 1, 0))
+    d = rng.uniform(1.0, 2.0, size=n)
+    u = rng.uniform(-ratio, ratio, size=max(n - 1, 0)) * d[1:]
+    # UB is tridiagonal: (UB)[i,i] = d_i lam_i + u_i nu_i, (UB)[i+1,i] = d_{i+1} nu_i,
+    # (UB)[i,i+1] = u_i lam_{i+1}.
+    x = np.zeros((n, n), dtype=np.float64, order="F")
+    for j in range(n):
+        col = np.zeros(min(j + 2, n))
+        col[j] = d[j] * lam[j] + (u[j] * nu[j] if j < n - 1 else 0.0)
+        if j + 1 < n:
+            col[j + 1] = d[j + 1] * nu[j]
+        if j >= 1:
+            col[j - 1] = u[j - 1] * lam[j]
+            col[:j + 1] -= u[j - 1] * x[:j + 1, j - 1]
+        x[:col.size, j] = col / d[j]
+    return x
+
+
+def cluster_spectrum(n: int, m: int, n_cluster: int, seed: int,
+                     spacing: float = 1e-8) -> tuple[np.ndarray, np.ndarray]:
+    """Spectrum with ``n_cluster`` eigenvalues at relative spacing ``spacing``.
+
+    Cluster members are interleaved with well separated eigenvalues so that
+    the bidiagonal B stays far from a Jordan block.  Returns (lam, shifts)
+    where shifts are the m prescribed eigenvalues (cluster first) perturbed by
+    relative 1e-12.
+    """
+    rng = np.random.default_rng(seed)
+    lam = rng.uniform(-np.sqrt(n), np.sqrt(n), size=n)
+    lam_c = 0.5 * np.sqrt(n) + 1.0
+    stride = max(1, n // max(n_cluster, 1))
+    pos = np.arange(n_cluster) * stride
+    lam[pos] = lam_c * (1.0 + spacing * np.arange(n_cluster))
+    rest = np.setdiff1d(np.arange(n), pos)
+    chosen = np.concatenate([pos, rng.choice(rest, size=m - n_cluster, replace=False)])
+    shifts = lam[chosen] * (1.0 + SHIFT_PERTURBATION)
+    return lam, shifts
+
+
+def real_shifts(h: np.ndarray, m: int) -> np.ndarray:
+    """First m real eigenvalues (dgeev order), perturbed by relative 1e-12."""
+    w = np.linalg.eigvals(np.asarray(h))
+    real = w.real[w.imag == 0.0]
+    if real.size < m:
+        raise ValueError(f"matrix has only {real.size} real eigenvalues, {m} requested")
+    return np.ascontiguousarray(real[:m] * (1.0 + SHIFT_PERTURBATION))
+
+
+# ---------------------------------------------------------------------------
+# Named workloads (sizes of the benchmark configurations)
+# ---------------------------------------------------------------------------
+
+def batch_sizes(count: int = 256, seed: int = 3, nmin: int = 200, nmax: int = 4000):
+    """(n_p, m_p, graded_p, seed_p) for the batched workload."""
+    rng = np.random.default_rng(seed)
+    n = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count))).astype(int)
+    f = rng.uniform(0.05, 0.2, size=count)
+    m = np.maximum(1, np.round(f * n)).astype(int)
+    g = rng.random(count) < 0.10
+    seeds = 1000 + np.arange(count)
+    return [(int(a), int(b), bool(c), int(d)) for a, b, c, d in zip(n, m, g, seeds)]
+
+
+def load_shifts(name: str) -> np.ndarray:
+    path = os.path.join(GOLDEN_DIR, name)
+    with np.load(path, allow_pickle=False) as z:
+        return np.array(z["shifts"])

@@ -1,0 +1,285 @@
+"""hsrq3in_amd -- robust tiled inverse iteration for upper Hessenberg matrices
+on one MI355X (HIP / gfx950), behind the reference's C entry point.
+
+The compute path is libhsrq.so (HIP kernels + C-ABI, ``include/hsrq.h`` and
+``include/inverse_iteration.h``).  This module is a thin ctypes binding:
+
+* ``tiled_inverse_iteration(H, wr, tlsz, rhs_tlsz)`` -- the reference's
+  entry point (reference inverse_iteration.h:19-22) on host arrays, through
+  the drop-in symbol of the same name;
+* ``Context`` -- device-resident solves, batches, reusable plans (HIP-graph
+  capture), on-device residuals;
+* ``Server`` (``hsrq3in_amd.serve``) -- continuous-batching request loop.
+
+There is no fallback: if libhsrq.so is missing or fails to load, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhsrq.so")
+
+HSRQ_CONVERGED = 1
+HSRQ_PERTURBED = 2
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class HsrqError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        msg = lib().hsrq_strerror(code).decode() if code < 0 else str(code)
+        super().__init__(f"{what}: {msg} ({code})" if what else f"{msg} ({code})")
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("H", _vp), ("ldH", ctypes.c_int), ("m", ctypes.c_int),
+                ("wr", _vp), ("vr", _vp), ("ldvr", ctypes.c_int)]
+
+
+# Every symbol the C-ABI exports (checked by the CPU test-suite).
+EXPORTS = [
+    "tiled_inverse_iteration",
+    "hsrq_context_create", "hsrq_context_destroy", "hsrq_context_set_tile",
+    "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_strerror",
+    "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
+    "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
+    "hsrq_plan_sync", "hsrq_plan_status",
+    "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
+    "hsrq_residuals_device",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libhsrq.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m hsrq3in_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.tiled_inverse_iteration.restype = None
+        L.tiled_inverse_iteration.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                              ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+        L.hsrq_strerror.restype = ctypes.c_char_p
+        L.hsrq_strerror.argtypes = [ctypes.c_int]
+        L.hsrq_context_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int]
+        L.hsrq_context_destroy.restype = None
+        L.hsrq_context_destroy.argtypes = [_vp]
+        L.hsrq_context_set_tile.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_set_exact_norms.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_set_precision.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_stream.restype = _vp
+        L.hsrq_context_stream.argtypes = [_vp]
+        L.hsrq_selftest.argtypes = [_vp]
+        L.hsrq_solve_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp,
+                                        _vp, ctypes.c_int, ctypes.c_int, _ip]
+        L.hsrq_solve_batch_device.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem),
+                                              ctypes.c_int, _ip]
+        L.hsrq_plan_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem), ctypes.c_int,
+                                       ctypes.POINTER(_vp)]
+        for f in ("hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_sync"):
+            getattr(L, f).argtypes = [_vp]
+        L.hsrq_plan_execute_profiled.argtypes = [_vp, _dp, _ip]
+        L.hsrq_plan_status.argtypes = [_vp, _ip]
+        L.hsrq_plan_workspace_bytes.restype = ctypes.c_size_t
+        L.hsrq_plan_workspace_bytes.argtypes = [_vp]
+        L.hsrq_plan_destroy.restype = None
+        L.hsrq_plan_destroy.argtypes = [_vp]
+        L.hsrq_residuals_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                            _vp, _vp, ctypes.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise HsrqError(rc, what)
+
+
+# ---------------------------------------------------------------------------
+# Reference interface (host arrays)
+# ---------------------------------------------------------------------------
+
+def tiled_inverse_iteration(H: np.ndarray, wr: np.ndarray, tlsz: int = 64, rhs_tlsz: int = 32,
+                            vr: np.ndarray | None = None):
+    """Eigenvectors of the upper Hessenberg H for the real shifts wr.
+
+    Mirrors the reference's C entry point (inverse_iteration.h:19-22): H is
+    n x n column-major (a Fortran-ordered array), wr is updated in place
+    (reordered, converged first, if restarts were needed) and returned with
+    the n x m unit-norm eigenvector matrix vr.
+    """
+    H = np.asfortranarray(H, dtype=np.float64)
+    n = H.shape[0]
+    if H.ndim != 2 or H.shape[1] != n:
+        raise ValueError("H must be square")
+    wr = np.ascontiguousarray(wr, dtype=np.float64)
+    m = wr.size
+    if vr is None:
+        vr = np.zeros((n, m), dtype=np.float64, order="F")
+    if not (vr.flags.f_contiguous and vr.shape == (n, m)):
+        raise ValueError("vr must be a Fortran-ordered n x m array")
+    lib().tiled_inverse_iteration(n, H.ctypes.data_as(_dp), n, m, wr.ctypes.data_as(_dp), tlsz,
+                                  rhs_tlsz, vr.ctypes.data_as(_dp), n)
+    return vr, wr
+
+
+# ---------------------------------------------------------------------------
+# Device API (torch tensors as device memory; column-major = transposed view)
+# ---------------------------------------------------------------------------
+
+def colmajor_to_device(a: np.ndarray, device="cuda"):
+    """Fortran-ordered numpy matrix -> torch tensor T with T[j, i] = a[i, j]
+    (the column-major layout the C-ABI expects; leading dimension a.shape[0])."""
+    import torch
+    a = np.asfortranarray(a, dtype=np.float64)
+    return torch.from_numpy(a.T).to(device)
+
+
+def colmajor_to_host(t) -> np.ndarray:
+    """Inverse of colmajor_to_device."""
+    return np.asfortranarray(t.detach().cpu().numpy().T)
+
+
+class Context:
+    """Device, stream and settings of libhsrq (hsrq_context)."""
+
+    def __init__(self, device: int = 0, tile: int = 0, exact_norms: bool = False):
+        self._h = _vp()
+        _check(lib().hsrq_context_create(ctypes.byref(self._h), device), "hsrq_context_create")
+        if tile:
+            _check(lib().hsrq_context_set_tile(self._h, tile), "set_tile")
+        if exact_norms:
+            _check(lib().hsrq_context_set_exact_norms(self._h, 1), "set_exact_norms")
+
+    def close(self):
+        if self._h:
+            lib().hsrq_context_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_exact_norms(self, on: bool):
+        _check(lib().hsrq_context_set_exact_norms(self._h, int(on)), "set_exact_norms")
+
+    def set_precision(self, prec: int):
+        _check(lib().hsrq_context_set_precision(self._h, prec), "set_precision")
+
+    def selftest(self):
+        _check(lib().hsrq_selftest(self._h), "hsrq_selftest")
+
+    def stream_ptr(self) -> int:
+        return lib().hsrq_context_stream(self._h) or 0
+
+    def solve(self, Ht, wrt, vrt, tlsz: int = 0):
+        """Solve on device tensors (column-major via transposed storage):
+        Ht (n, ldH) with Ht[j, i] = H(i, j); wrt (m,); vrt (m, ldvr).
+        Returns the per-shift status array."""
+        n = Ht.shape[0]
+        m = wrt.numel()
+        st = np.zeros(max(m, 1), dtype=np.int32)
+        rc = lib().hsrq_solve_device(self._h, n, Ht.data_ptr(), Ht.shape[1], m, wrt.data_ptr(),
+                                     vrt.data_ptr(), vrt.shape[1], tlsz,
+                                     st.ctypes.data_as(_ip))
+        _check(rc, "hsrq_solve_device")
+        return st[:m]
+
+    @staticmethod
+    def _problems(items):
+        arr = (Problem * len(items))()
+        for i, (Ht, wrt, vrt) in enumerate(items):
+            arr[i] = Problem(Ht.shape[0], Ht.data_ptr(), Ht.shape[1], wrt.numel(), wrt.data_ptr(),
+                             vrt.data_ptr(), vrt.shape[1])
+        return arr
+
+    def solve_batch(self, items, tlsz: int = 0):
+        """items: list of (Ht, wrt, vrt). Returns one status array per problem."""
+        arr = self._problems(items)
+        total = sum(w.numel() for _, w, _ in items)
+        st = np.zeros(max(total, 1), dtype=np.int32)
+        _check(lib().hsrq_solve_batch_device(self._h, len(items), arr, tlsz,
+                                             st.ctypes.data_as(_ip)), "hsrq_solve_batch_device")
+        out, pos = [], 0
+        for _, w, _ in items:
+            out.append(st[pos:pos + w.numel()])
+            pos += w.numel()
+        return out
+
+    def plan(self, items, tlsz: int = 0) -> "Plan":
+        return Plan(self, items, tlsz)
+
+    def residuals(self, Ht, wrt, vrt) -> np.ndarray:
+        """||(H - wr_k I) x_k|| / (||H||_oo ||x_k|| 2^-52) per eigenvector (device)."""
+        m = wrt.numel()
+        res = np.zeros(max(m, 1), dtype=np.float64)
+        _check(lib().hsrq_residuals_device(self._h, Ht.shape[0], Ht.data_ptr(), Ht.shape[1], m,
+                                           wrt.data_ptr(), vrt.data_ptr(), vrt.shape[1],
+                                           res.ctypes.data_as(_dp)), "hsrq_residuals_device")
+        return res[:m]
+
+
+class Plan:
+    """Reusable device schedule for a set of problems (hsrq_plan)."""
+
+    def __init__(self, ctx: Context, items, tlsz: int = 0):
+        self.ctx = ctx
+        self.items = items
+        self._arr = Context._problems(items)
+        self._h = _vp()
+        _check(lib().hsrq_plan_create(ctx.handle, len(items), self._arr, tlsz,
+                                      ctypes.byref(self._h)), "hsrq_plan_create")
+        self.total_m = sum(w.numel() for _, w, _ in items)
+
+    def execute(self, graph: bool = False):
+        f = lib().hsrq_plan_execute_graph if graph else lib().hsrq_plan_execute
+        _check(f(self._h), "hsrq_plan_execute")
+
+    KERNELS = ("norms", "init", "diag", "update", "backtransform")
+
+    def execute_profiled(self) -> dict:
+        """Synchronous run with HIP events per launch: {kernel: (ms, launches)}."""
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
+        _check(lib().hsrq_plan_execute_profiled(self._h, ms, cnt), "hsrq_plan_execute_profiled")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNELS)}
+
+    def sync(self):
+        _check(lib().hsrq_plan_sync(self._h), "hsrq_plan_sync")
+
+    def status(self) -> np.ndarray:
+        st = np.zeros(max(self.total_m, 1), dtype=np.int32)
+        _check(lib().hsrq_plan_status(self._h, st.ctypes.data_as(_ip)), "hsrq_plan_status")
+        return st[:self.total_m]
+
+    def workspace_bytes(self) -> int:
+        return int(lib().hsrq_plan_workspace_bytes(self._h))
+
+    def close(self):
+        if self._h:
+            lib().hsrq_plan_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

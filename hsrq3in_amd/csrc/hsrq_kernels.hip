@@ -1,0 +1,843 @@
+// hsrq_kernels.hip -- gfx950 kernels of the robust tiled inverse iteration.
+//
+// One problem is one fused right-to-left sweep over tile columns J:
+//   k_diag(J)    one wavefront per shift: Givens generation on the diagonal
+//                tile fused with the robust back substitution (reference
+//                reduce.c:27-68 + solve-task.c:19-225), then the coefficient
+//                vectors W_J (cross-over reduction) and Z_J (update) and the
+//                per-(tile row, shift) scaling exponents of the update;
+//   k_update(J)  FP64 MFMA: [R~_out | dY](0:l) = H(0:l, l-1 : l+b-2) [W | Z]
+//                with a scaled epilogue that adds the cross-over terms and
+//                updates Y (reference reduce.c:73-143 + update-task.c:16-262
+//                in one pass over the panel, all shifts at once);
+//   k_backtransform  x = Q^T y, 2-norm, convergence, normalisation
+//                (shifted_Hessenberg_solve.c:104-245) as an affine scan.
+// Work is described by item tables, so one launch serves any number of
+// problems (batching and continuous batching use the same kernels).
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "hsrq_device.h"
+#include "hsrq_internal.h"
+
+namespace hsrq {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// K0: tile norms.  rowinf[I] = ||H(I, :)||_oo (all n columns, as the
+// reference's dlange('I', n, n, H) of inverse_iteration.c:154), diagn[I] =
+// ||H(I, I)||_oo of the b x b diagonal tile (solve-task.c:30-34).
+// One workgroup per (problem, tile); one thread per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_norms(const ProbDev* __restrict__ probs,
+                                               const RowItem* __restrict__ items) {
+  const RowItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int I = it.rb;
+  const int l = tile_first_row(P, I), nb = tile_rows(P, I);
+  const int r = threadIdx.x;
+  double rs = 0.0, ds = 0.0;
+  if (r < nb) {
+    const double* h = P.H + l + r;
+    const int64_t ld = P.ldH;
+    int j = 0;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (; j + 4 <= P.n; j += 4) {
+      a0 += fabs(h[(j + 0) * ld]);
+      a1 += fabs(h[(j + 1) * ld]);
+      a2 += fabs(h[(j + 2) * ld]);
+      a3 += fabs(h[(j + 3) * ld]);
+    }
+    for (; j < P.n; ++j) a0 += fabs(h[j * ld]);
+    rs = (a0 + a1) + (a2 + a3);
+    for (int c = 0; c < nb; ++c) ds += fabs(h[(int64_t)(l + c) * ld]);
+  }
+  __shared__ double red[2][4];
+  const int wave = threadIdx.x >> 6;
+  const double mr = wave_max(rs), md = wave_max(ds);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][wave] = mr;
+    red[1][wave] = md;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.rowinf[I] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    P.diagn[I] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Init: starting vectors (reference inverse_iteration.c:19-57), initial
+// cross-over columns H(:, n-1) - mu e_{n-1} (reduce.c:239-246), exponents 0,
+// per-row-block norms.  Workgroup = (problem, row block, 4 shifts).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
+                                              const RowItem* __restrict__ items) {
+  const RowItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = it.k0 + wave;
+  if (k >= P.m) return;  // whole wavefront exits; no barriers below
+  double hinf = 0.0;
+  for (int t = lane; t < P.T; t += 64) hinf = fmax(hinf, P.rowinf[t]);
+  hinf = wave_max(hinf);
+  const double eps3 = hinf * 0x1p-53;   // inverse_iteration.c:157-158
+  if (it.rb == 0 && k == 0 && lane == 0) {
+    P.scal[0] = hinf;
+    P.scal[1] = eps3;
+  }
+  int tile, r0, nr;
+  rowblock_range(P, it.rb, tile, r0, nr);
+  const int n = P.n;
+  double yv = 0.0;
+  if (lane < nr) {
+    const int row = r0 + lane;
+    if (P.its < 0) {
+      yv = eps3;
+    } else {
+      const double sq = sqrt((double)n);
+      yv = (row == 0) ? eps3 : eps3 / (sq + 1.0);
+      if (row == n - 1 - P.its) yv -= eps3 * sq;
+    }
+    P.Y[row + (int64_t)k * P.ldY] = yv;
+    double h = P.H[row + (int64_t)(n - 1) * P.ldH];
+    if (row == n - 1) h -= P.wr[k];
+    P.Rt[(P.T - 1) & 1][row + (int64_t)k * n] = h;
+  }
+  const double mx = wave_max(fabs(yv));
+  if (lane == 0) P.Ynb[(int64_t)it.rb * P.m + k] = mx;
+  if (it.rb == 0) {
+    for (int t = lane; t < P.T; t += 64) P.E[(int64_t)t * P.m + k] = 0;
+    if (lane == 0) P.status[k] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1: diagonal tile J, one wavefront per shift, RPL rows per lane.
+// ---------------------------------------------------------------------------
+template <int RPL>
+__device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
+  double t = a[0];
+#pragma unroll
+  for (int i = 1; i < RPL; ++i)
+    if (q == i) t = a[i];
+  return t;
+}
+
+template <int RPL>
+__global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
+                                              const K1Item* __restrict__ items) {
+  constexpr int BMAX = 64 * RPL;
+  constexpr int CW = (RPL == 4) ? 32 : 64;   // tile columns per LDS chunk
+  __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
+
+  const K1Item it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int J = it.J;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m = P.m, n = P.n;
+  const int kraw = it.k0 + wave;
+  const bool valid = kraw < m;
+  const int k = valid ? kraw : m - 1;
+  const int l = tile_first_row(P, J), nb = tile_rows(P, J);
+  const double mu = P.wr[k];
+  const double* __restrict__ H = P.H;
+  const int64_t ldH = P.ldH;
+  const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
+  double* yg = P.Y + (int64_t)k * P.ldY + l;
+
+  double v[RPL], y[RPL], cr[RPL], sr[RPL];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = lane + 64 * q;
+    v[q] = r < nb ? Rin[r] : 0.0;
+    y[q] = r < nb ? yg[r] : 0.0;
+    cr[q] = 1.0;
+    sr[q] = 0.0;
+  }
+  int E = P.E[(int64_t)J * m + k];
+  const double Hnorm = P.diagn[J];
+  const double eps = 0x1p-53;
+  const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
+  const double smin = fmax(eps * fabs(mu), smlnum);                        // solve-task.c:47
+  int info = 0;
+  const double slack = 1.0 + 0x1p-40;
+  const bool exact = P.force_exact != 0;
+
+  // Upper bounds of |y| and |v| over the rows still active (<= j).
+  double By, Bv;
+  {
+    double ly = 0.0, lv = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      ly = fmax(ly, fabs(y[q]));
+      lv = fmax(lv, fabs(v[q]));
+    }
+    By = wave_max(ly);
+    Bv = wave_max(lv);
+  }
+
+  int cur = -1;
+  for (int j = nb - 1; j >= 1; --j) {
+    const int col = j - 1;
+    const int ch = col / CW;
+    if (ch != cur) {
+      __syncthreads();
+      const int c0 = ch * CW;
+      const int nc = min(CW, nb - c0);
+      for (int idx = threadIdx.x; idx < nc * BMAX; idx += 256) {
+        const int c = idx / BMAX, r = idx - c * BMAX;
+        Hs[c * BMAX + r] = (r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+      }
+      __syncthreads();
+      cur = ch;
+    }
+    const double* hc = &Hs[(col - ch * CW) * BMAX];
+    const int jl = j & 63, jq = j >> 6;
+
+    // Rotation G_j from the running column (reduce.c:45 / solve-task.c:171).
+    const double vj = readlane(pick<RPL>(v, jq), jl);
+    const double hsub = hc[j];
+    double c, s;
+    givens(vj, hsub, c, s);
+    if (lane == jl) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (q == jq) {
+          cr[q] = c;
+          sr[q] = s;
+        }
+    }
+
+    // y(j) = y(j) / (c v(j) - s H(j,j-1))  (solve-task.c:69-75).
+    double yj = readlane(pick<RPL>(y, jq), jl);
+    const int e1 = solve_1x1(smin, c * vj, s * hsub, yj, info);
+    if (e1 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e1);
+      E += e1;
+      By = ldexp(By, e1);
+    }
+    if (lane == jl) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (q == jq) y[q] = yj;
+    }
+
+    // Protection of y(j-1) -= ... (solve-task.c:78-82), bounds first.
+    const double hdd = hc[j - 1] - mu;
+    double p1 = exact ? 0.0 : protect_double_update(Bv, fabs(yj), fabs(hdd), fabs(yj), By);
+    double yjm1 = 0.0, vjm1 = 0.0;
+    bool have_jm1 = false;
+    if (p1 != 1.0) {
+      const int il = (j - 1) & 63, iq = (j - 1) >> 6;
+      vjm1 = fabs(readlane(pick<RPL>(v, iq), il));
+      yjm1 = fabs(readlane(pick<RPL>(y, iq), il));
+      have_jm1 = true;
+      p1 = protect_double_update(vjm1, fabs(yj), fabs(hdd), fabs(yj), yjm1);
+    }
+    const int e2 = pow2_exp(p1);
+    const double yj2 = ldexp(yj, e2);
+
+    // Protection of y(0:j-2) -= ... (solve-task.c:84-89), bounds first.
+    double p2 = exact ? 0.0
+                      : protect_double_update(Bv, fabs(yj2), Hnorm, fabs(yj2), ldexp(By, e2));
+    double ynorm_x = -1.0, vnorm_x = -1.0;
+    if (p2 != 1.0) {
+      double ly = 0.0, lv = 0.0;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const int r = lane + 64 * q;
+        if (r < j - 1) {
+          ly = fmax(ly, fabs(y[q]));
+          lv = fmax(lv, fabs(v[q]));
+        }
+      }
+      ynorm_x = wave_max(ly);
+      vnorm_x = wave_max(lv);
+      p2 = protect_double_update(vnorm_x, fabs(yj2), Hnorm, fabs(yj2), ldexp(ynorm_x, e2));
+    }
+    const int e3 = pow2_exp(p2);
+    const int et = e2 + e3;
+    if (et != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], et);
+      E += et;
+      yj = ldexp(yj, et);
+    }
+
+    // Updates of y(0:j-1) and of the annihilation column v(0:j-1)
+    // (solve-task.c:82, :91-93, :98-101).
+    const double yc = yj * c, ys = yj * s;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      if (r <= j - 1) {
+        const double h = (r == j - 1) ? hdd : hc[r];
+        y[q] = y[q] - yc * v[q] + ys * h;
+        v[q] = c * h + s * v[q];
+      }
+    }
+
+    // Bounds for the rows that stay active (<= j-2 after this step): exact
+    // when the exact path was taken, else propagated through the update.
+    if (have_jm1 || ynorm_x >= 0.0) {
+      double ly = 0.0, lv = 0.0;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const int r = lane + 64 * q;
+        if (r <= j - 2) {
+          ly = fmax(ly, fabs(y[q]));
+          lv = fmax(lv, fabs(v[q]));
+        }
+      }
+      By = wave_max(ly);
+      Bv = wave_max(lv);
+    } else {
+      const double Hm = fmax(Hnorm, fabs(hdd));
+      const double Byp = ldexp(By, et);
+      By = (Byp + fabs(yj) * (fabs(c) * Bv + fabs(s) * Hm)) * slack;
+      Bv = (fabs(c) * Hm + fabs(s) * Bv) * slack;
+    }
+  }
+
+  // Final rotation and y(0) (solve-task.c:104-118, :212-218).
+  {
+    double v0 = readlane(v[0], 0);
+    if (J > 0) {
+      const double hleft = H[l + (int64_t)(l - 1) * ldH];
+      double c0 = 1.0, s0 = 0.0;
+      if (hleft != 0.0) givens(v0, hleft, c0, s0);
+      v0 = v0 * c0 - hleft * s0;
+      if (lane == 0) {
+        cr[0] = c0;
+        sr[0] = s0;
+      }
+    }
+    double y0 = readlane(y[0], 0);
+    const int e = solve_1x1(smin, v0, 0.0, y0, info);
+    if (e != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e);
+      E += e;
+    }
+    if (lane == 0) y[0] = y0;
+  }
+
+  if (valid) {
+    double* Cg = P.C + (int64_t)k * n + l;
+    double* Sg = P.S + (int64_t)k * n + l;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      if (r < nb) {
+        yg[r] = y[q];
+        Cg[r] = cr[q];
+        Sg[r] = sr[q];
+      }
+    }
+    if (lane == 0) {
+      P.E[(int64_t)J * m + k] = E;
+      if (info) atomicOr(&P.status[k], 2);
+    }
+  }
+  if (J == 0) return;
+
+  // ---- Coefficients of the panel pass (J >= 1, nb = b, full tile) ----
+  // W: w_r = c_r prod_{i<r} s_i,  P = prod_{i<b} s_i.
+  // Z: t_r = s_r t_{r-1} + c_r y_r (t_{-1} = 0);  Z_0 = -s_0 y_0,
+  //    Z_r = c_r t_{r-1} - s_r y_r (1 <= r < b),  Zlast = t_{b-1}
+  //    (update-task.c:109-117, :187-198).
+  double w[RPL], z[RPL];
+  double carry_p = 1.0, carry_t = 0.0;
+  double zmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = lane + 64 * q;
+    const bool in = r < nb;
+    const double sq = in ? sr[q] : 1.0;
+    const double cq = in ? cr[q] : 0.0;
+    // exclusive prefix product of s
+    const double inc = wave_prefix_prod(sq);
+    double ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = 1.0;
+    w[q] = cq * (ex * carry_p);
+    carry_p *= readlane(inc, 63);
+    // affine scan for t
+    double a = in ? sq : 1.0, bb = in ? cq * y[q] : 0.0;
+    wave_affine_scan(a, bb);
+    const double t = a * carry_t + bb;   // t_r
+    double tprev = __shfl_up(t, 1, 64);
+    if (lane == 0) tprev = carry_t;
+    if (r == 0)
+      z[q] = -sq * y[q];
+    else
+      z[q] = in ? cq * tprev - sq * y[q] : 0.0;
+    carry_t = readlane(t, 63);
+    zmax = fmax(zmax, fabs(z[q]));
+  }
+  const double Pk = carry_p;
+  const double zlast = carry_t;   // rows >= nb are identity maps
+  const double zn = fmax(wave_max(zmax), fabs(zlast));
+
+  // Prescale so that the panel product cannot overflow:
+  // |acc| <= 2 (||H||_oo + |mu|) ||Z||_oo must stay below 2^1020.
+  const double hinf = P.scal[0];
+  const double tmax = 2.0 * (hinf + fabs(mu));
+  int p = 0;
+  if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
+
+  if (valid) {
+    double* Bw = P.Bm + (int64_t)(2 * k) * P.b;
+    double* Bz = Bw + P.b;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      if (r < nb) {
+        Bw[r] = w[q];
+        Bz[r] = ldexp(z[q], -p);
+      }
+    }
+    if (lane == 0) {
+      P.aux[2 * k] = Pk;
+      P.aux[2 * k + 1] = ldexp(zlast, -p);
+    }
+    // Scaling exponents of the update of every tile row I < J: consolidate
+    // (update-task.c:68-88), then one protection for the whole fused update.
+    // |update(i)| <= (||H(I,:)||_oo + |mu|) ||Z|| + |R~(i)| |Zlast| and
+    // |R~(i)| <= ||(H - mu I)(i,:)||_2, so t_I = 2 (||H(I,:)||_oo + |mu|).
+    const int EJ = E;
+    for (int I = lane; I < J; I += 64) {
+      const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
+      double yn = 0.0;
+      for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, P.Ynb[(int64_t)rb * m + k]);
+      const int EI = P.E[(int64_t)I * m + k];
+      const int c0 = min(EI, EJ);
+      const double yI = ldexp(yn, c0 - EI);
+      const double zJ = ldexp(zn, c0 - EJ);
+      const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
+      const int ea = pow2_exp(protect_update(tI, zJ, yI));
+      const int ec = c0 + ea;
+      P.SY[(int64_t)I * m + k] = ec - EI;
+      P.SZ[(int64_t)I * m + k] = ec - EJ + p;
+      P.E[(int64_t)I * m + k] = ec;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K2/K3: panel pass for tile column J on FP64 MFMA (v_mfma_f64_16x16x4_f64).
+// Block = 64 rows (one row block, inside one tile row I) x 64 columns of
+// [W | Z] interleaved (32 shifts).  Four waves, 32 x 32 each.
+// ---------------------------------------------------------------------------
+constexpr int KC = 32;   // K chunk staged in LDS
+constexpr int LDA = 64 + 2;
+constexpr int LDB = 64 + 2;
+
+__global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
+                                                const GemmItem* __restrict__ items) {
+  __shared__ double As[KC * LDA];   // As[k * LDA + row]
+  __shared__ double Bs[KC * LDB];   // Bs[k * LDB + col]
+  __shared__ double red[2][64];
+
+  const GemmItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int J = it.J;
+  const int m = P.m, n = P.n, b = P.b;
+  const int l = tile_first_row(P, J);
+  int tile, r0, nr;
+  rowblock_range(P, it.rb, tile, r0, nr);
+  const int col0 = it.cb * kGemmCols;
+  const int ncols = min(kGemmCols, 2 * m - col0);
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * P.ldH;   // panel column 0 = l-1
+  const double* __restrict__ Bg = P.Bm;
+
+  double4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+  for (int kc = 0; kc < b; kc += KC) {
+#pragma unroll
+    for (int i = 0; i < (KC * 64) / 256; ++i) {
+      const int idx = tid + 256 * i;
+      const int kk = idx >> 6, rr = idx & 63;
+      As[kk * LDA + rr] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * P.ldH] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < (KC * 64) / 256; ++i) {
+      const int idx = tid + 256 * i;
+      const int cc = idx / KC, kk = idx - cc * KC;
+      Bs[kk * LDB + cc] = (cc < ncols) ? Bg[(kc + kk) + (int64_t)(col0 + cc) * b] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kr = k4 + (lane >> 4);
+      double av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) av[a] = As[kr * LDA + wm * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) bv[c] = Bs[kr * LDB + wn * 32 + c * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue.  v_mfma_f64_16x16x4_f64 result layout: D[i][j] is register
+  // i / 4 of lane j + 16 (i % 4), so acc[a][c][r] holds
+  //   row wm*32 + a*16 + 4*r + lane/16,  col wn*32 + c*16 + lane%16.
+  const double* __restrict__ Rin = P.Rt[J & 1];
+  double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+  double cmax[2] = {0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int cl = wn * 32 + c * 16 + (lane & 15);
+    const int col = col0 + cl;
+    if (cl < ncols) {
+      const int k = col >> 1;
+      const bool isz = col & 1;
+      const double mu = P.wr[k];
+      const double coef = P.aux[2 * k + (isz ? 1 : 0)];
+      const double b0 = Bg[(int64_t)col * b];
+      const int sy = P.SY[(int64_t)tile * m + k];
+      const int sz = P.SZ[(int64_t)tile * m + k];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm * 32 + a * 16 + 4 * r + (lane >> 4);
+          if (row < nr) {
+            const int gi = r0 + row;
+            const double rin = Rin[gi + (int64_t)k * n];
+            double val = acc[a][c][r] + coef * rin;
+            if (gi == l - 1) val -= mu * b0;
+            if (!isz) {
+              Rout[gi + (int64_t)k * n] = val;
+            } else {
+              double* yp = P.Y + gi + (int64_t)k * P.ldY;
+              const double ynew = ldexp(*yp, sy) - ldexp(val, sz);
+              *yp = ynew;
+              cmax[c] = fmax(cmax[c], fabs(ynew));
+            }
+          }
+        }
+      }
+    }
+  }
+  // Column maxima over the block's rows: lanes l, l+16, l+32, l+48 share a
+  // column; then the two row-waves through LDS.
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double x = cmax[c];
+    x = fmax(x, __shfl_xor(x, 16, 64));
+    x = fmax(x, __shfl_xor(x, 32, 64));
+    cmax[c] = x;
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) red[wm][wn * 32 + c * 16 + lane] = cmax[c];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int cl = tid;
+    const int col = col0 + cl;
+    if (cl < ncols && (col & 1)) {
+      const int k = col >> 1;
+      P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: back-transform x = Q^T y with tiles brought to the common (smallest)
+// scale, three-accumulator 2-norm, convergence test and normalisation
+// (shifted_Hessenberg_solve.c:104-245).  The rotation chain
+//   tau1_i = s_i tau1_{i-1} + c_i tau2_i,  x_{i-1} = c_i tau1_{i-1} - s_i tau2_i
+// is an affine recurrence: chunk compositions + workgroup scan.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict__ probs,
+                                                       const ShiftItem* __restrict__ items) {
+  __shared__ int emin_s[4];
+  __shared__ double scanA[4], scanB[4];
+  __shared__ double accs[3][4];
+  __shared__ double ynrm_s;
+
+  const ShiftItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int k = it.k;
+  const int n = P.n, m = P.m, T = P.T;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+  // smallest exponent over the tiles (reduce_scaling_factors, robust.c:58-82)
+  int em = 0x7fffffff;
+  for (int t = tid; t < T; t += 256) em = min(em, P.E[(int64_t)t * m + k]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) em = min(em, __shfl_xor(em, off, 64));
+  if (lane == 0) emin_s[wave] = em;
+  __syncthreads();
+  const int emin = min(min(emin_s[0], emin_s[1]), min(emin_s[2], emin_s[3]));
+
+  double* x = P.Y + (int64_t)k * P.ldY;
+  const double* c = P.C + (int64_t)k * n;
+  const double* s = P.S + (int64_t)k * n;
+  const int CH = (n + 255) / 256;
+  const int i0 = min(n, tid * CH), i1 = min(n, i0 + CH);
+
+  auto alpha_of = [&](int i) -> double {
+    const int t = (i < P.b0) ? 0 : 1 + (i - P.b0) / P.b;
+    return ldexp(1.0, emin - P.E[(int64_t)t * m + k]);
+  };
+
+  // pass 1: composition of the chunk's maps
+  double A = 1.0, B = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const double t2 = alpha_of(i) * x[i];
+    const double a = (i == 0) ? 0.0 : s[i];
+    const double bb = (i == 0) ? t2 : c[i] * t2;
+    B = a * B + bb;
+    A = a * A;
+  }
+  const double ylast = (i1 > i0) ? x[i1 - 1] : 0.0;
+  // workgroup exclusive scan of the maps
+  double a = A, bb = B;
+  wave_affine_scan(a, bb);
+  if (lane == 63) {
+    scanA[wave] = a;
+    scanB[wave] = bb;
+  }
+  double ea = __shfl_up(a, 1, 64), eb = __shfl_up(bb, 1, 64);
+  if (lane == 0) {
+    ea = 1.0;
+    eb = 0.0;
+  }
+  __syncthreads();
+  double pa = 1.0, pb = 0.0;   // composite of the previous waves
+  for (int w = 0; w < wave; ++w) {
+    pb = scanA[w] * pb + scanB[w];
+    pa = scanA[w] * pa;
+  }
+  double tau1 = ea * pb + eb;   // value entering this chunk
+
+  // pass 2: the chain itself
+  Acc3 acc = {0.0, 0.0, 0.0};
+  for (int i = i0; i < i1; ++i) {
+    const double yi = (i == i1 - 1) ? ylast : x[i];
+    const double t2 = alpha_of(i) * yi;
+    if (i == 0) {
+      tau1 = t2;
+    } else {
+      const double xi = c[i] * tau1 - s[i] * t2;
+      tau1 = c[i] * t2 + s[i] * tau1;
+      x[i - 1] = xi;
+      acc3_add(acc, xi);
+    }
+  }
+  if (i1 == n && i1 > i0) {
+    x[n - 1] = tau1;
+    acc3_add(acc, tau1);
+  }
+  const double s0 = wave_sum(acc.asml), s1 = wave_sum(acc.amed), s2 = wave_sum(acc.abig);
+  if (lane == 0) {
+    accs[0][wave] = s0;
+    accs[1][wave] = s1;
+    accs[2][wave] = s2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Acc3 t;
+    t.asml = (accs[0][0] + accs[0][1]) + (accs[0][2] + accs[0][3]);
+    t.amed = (accs[1][0] + accs[1][1]) + (accs[1][2] + accs[1][3]);
+    t.abig = (accs[2][0] + accs[2][1]) + (accs[2][2] + accs[2][3]);
+    const double ynrm = acc3_norm(t);
+    // converged iff the unscaled growth ||x|| 2^-emin exceeds 0.1/sqrt(n)
+    const bool conv = isfinite(ynrm) && ynrm > ldexp(0.1 / sqrt((double)n), emin);
+    ynrm_s = conv ? ynrm : -1.0;
+    if (conv) atomicOr(&P.status[k], 1);
+  }
+  __syncthreads();
+  const double ynrm = ynrm_s;
+  if (ynrm > 0.0)
+    for (int i = tid; i < n; i += 256) x[i] = x[i] / ynrm;
+}
+
+// ---------------------------------------------------------------------------
+// Validator (test / benchmark only): r_k = (H - wr_k I) x_k for every
+// eigenvector, then ||r_k||_2 / (||H||_oo ||x_k||_2 eps), eps = 2^-52, with
+// overflow-safe (scaled) 2-norms.  H is upper Hessenberg: row i starts at
+// column i-1 (the reference's dense product of validation.c:48-77 adds
+// zeros below the subdiagonal).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_resid_rows(int n, const double* __restrict__ H,
+                                                    int64_t ldH, const double* __restrict__ wr,
+                                                    const double* __restrict__ X, int64_t ldX,
+                                                    double* __restrict__ R) {
+  const int k = blockIdx.y;
+  const int row0 = blockIdx.x * 256;
+  const int i = row0 + threadIdx.x;
+  const double* x = X + (int64_t)k * ldX;
+  double acc = 0.0;
+  const int jstart = row0 > 0 ? row0 - 1 : 0;
+  if (i < n) {
+    for (int j = jstart; j < n; ++j) acc += H[i + (int64_t)j * ldH] * x[j];
+    acc -= wr[k] * x[i];
+    R[i + (int64_t)k * n] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rowsums(int n, const double* __restrict__ H, int64_t ldH,
+                                                 double* __restrict__ rs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    double a = 0.0;
+    for (int j = 0; j < n; ++j) a += fabs(H[i + (int64_t)j * ldH]);
+    rs[i] = a;
+  }
+}
+
+__device__ double block_max256(double v, double* sh) {
+  const int wave = threadIdx.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[wave] = v;
+  __syncthreads();
+  return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__device__ double block_sum256(double v, double* sh) {
+  const int wave = threadIdx.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[wave] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// Scaled 2-norm of a length-n vector by one workgroup.
+__device__ double block_nrm2(int n, const double* __restrict__ v, double* sh) {
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) mx = fmax(mx, fabs(v[i]));
+  mx = block_max256(mx, sh);
+  if (mx == 0.0 || !isfinite(mx)) return mx;
+  const int e = ilogb(mx);
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double t = ldexp(v[i], -e);
+    s += t * t;
+  }
+  s = block_sum256(s, sh);
+  return ldexp(sqrt(s), e);
+}
+
+__global__ __launch_bounds__(256) void k_resid_norms(int n, const double* __restrict__ rs,
+                                                     const double* __restrict__ X, int64_t ldX,
+                                                     const double* __restrict__ R,
+                                                     double* __restrict__ res) {
+  __shared__ double sh[4];
+  const int k = blockIdx.x;
+  double hmax = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) hmax = fmax(hmax, rs[i]);
+  hmax = block_max256(hmax, sh);
+  const double rn = block_nrm2(n, R + (int64_t)k * n, sh);
+  const double xn = block_nrm2(n, X + (int64_t)k * ldX, sh);
+  if (threadIdx.x == 0) res[k] = rn / (hmax * xn * 0x1p-52);
+}
+
+// ---------------------------------------------------------------------------
+// MFMA layout self-test: D = A (16x4) * B (4x16), A[i][k] = i + 100 k,
+// B[k][j] = (k == j % 4) ? 1 : 0  => D[i][j] = A[i][j % 4].  Operands: lane
+// i + 16 k holds A[i][k] and B[k][i]; result: see k_update's epilogue.
+// ---------------------------------------------------------------------------
+__global__ void k_mfma_selftest(int* __restrict__ bad) {
+  const int lane = threadIdx.x;
+  const double av = (double)(lane & 15) + 100.0 * (lane >> 4);
+  const double bv = ((lane >> 4) == ((lane & 15) & 3)) ? 1.0 : 0.0;
+  double4_t acc = (double4_t){0.0, 0.0, 0.0, 0.0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  int wrong = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = 4 * r + (lane >> 4), j = lane & 15;
+    const double want = (double)i + 100.0 * (j & 3);
+    if (acc[r] != want) wrong = 1;
+  }
+  if (wrong) atomicAdd(bad, 1);
+}
+
+}  // namespace hsrq
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (host side of this translation unit).
+// ---------------------------------------------------------------------------
+namespace hsrq {
+
+hipError_t launch_norms(const ProbDev* d_probs, const RowItem* items, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_norms, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_init(const ProbDev* d_probs, const RowItem* items, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(int rpl, const ProbDev* d_probs, const K1Item* items, int count,
+                       hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  switch (rpl) {
+    case 1: hipLaunchKernelGGL(k_diag<1>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    case 2: hipLaunchKernelGGL(k_diag<2>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    case 4: hipLaunchKernelGGL(k_diag<4>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update(const ProbDev* d_probs, const GemmItem* items, int count,
+                         hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_backtransform(const ProbDev* d_probs, const ShiftItem* items, int count,
+                                hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_backtransform, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
+                           const double* X, int64_t ldX, double* work, double* res,
+                           hipStream_t st) {
+  if (m <= 0) return hipSuccess;
+  double* R = work;
+  double* rs = work + (int64_t)n * m;
+  const int nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_rowsums, dim3(nb), dim3(256), 0, st, n, H, ldH, rs);
+  hipLaunchKernelGGL(k_resid_rows, dim3(nb, m), dim3(256), 0, st, n, H, ldH, wr, X, ldX, R);
+  hipLaunchKernelGGL(k_resid_norms, dim3(m), dim3(256), 0, st, n, (const double*)rs, X, ldX,
+                     (const double*)R, res);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_selftest(int* d_bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, st, d_bad);
+  return hipGetLastError();
+}
+
+}  // namespace hsrq

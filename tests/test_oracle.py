@@ -1,0 +1,232 @@
+"""Pins the CPU oracle against what fixes its answers independently:
+library routines (LAPACK dlartg via scipy, dense LU via numpy, triangular
+solves via scipy), closed forms (n = 1, 2; triangular H), invariants
+(exact similarity of graded matrices, rotation identities, norms) and
+brute-force properties of the robust primitives.  A plausible mistake in
+the restatement (wrong sign of s, dropped shift term, transposed operand,
+wrong ragged tile, missing rescale) fails at least one of these.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+from scipy.linalg import lapack
+
+import oracle
+from hsrq3in_amd import workloads as w
+from tests.helpers import case, cosines, one_minus_abs_cos
+
+EPS3_FACTOR = 2.0 ** -53
+OMEGA = 1e300
+
+
+def eps3_of(H):
+    return np.abs(H).sum(axis=1).max() * EPS3_FACTOR
+
+
+def dense_direction(H, s):
+    n = H.shape[0]
+    x = np.linalg.solve(H - s * np.eye(n), eps3_of(H) * np.ones(n))
+    return x / np.linalg.norm(x)
+
+
+# ---------------------------------------------------------------- rotations
+
+GRID = [0.0, 1.0, -1.0, 3.0, -4.0, 1e-300, -3e-310, 1e300, -7e307, 2.5e-160, 1e155, 0.7, -1e-20]
+
+
+@pytest.mark.parametrize("f", GRID)
+@pytest.mark.parametrize("g", GRID)
+def test_dlartg_matches_lapack(f, g):
+    c, s, r = oracle.dlartg(f, g)
+    cl, sl, rl = lapack.dlartg(f, g)
+    for a, b in ((c, cl), (s, sl), (r, rl)):
+        assert a == b or abs(a - b) <= 4e-16 * max(abs(a), abs(b)), (f, g, (c, s, r), (cl, sl, rl))
+
+
+def test_givens_convention():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        f, g = rng.standard_normal(2) * 10.0 ** rng.integers(-30, 30, size=2)
+        c, s = oracle.givens(f, g)
+        r = oracle.dlartg(f, g)[2]
+        # reference reduce.h:30-31 / reduce.c:23: transposed rotation
+        assert abs(c * c + s * s - 1.0) < 4e-16
+        assert abs(c * f - s * g - r) <= 4e-16 * abs(r)
+        assert abs(s * f + c * g) <= 4e-16 * abs(r)
+        assert c >= 0.0
+
+
+# ------------------------------------------------------- robust primitives
+
+def _log2(x):
+    return -math.inf if x == 0 else math.log2(x)
+
+
+def test_protect_update_properties():
+    """alpha*y + t*(alpha*x) never exceeds Omega, alpha = 1 whenever the
+    unscaled update is safe, alpha > 0 (brute force over exponent grids),
+    under the primitive's precondition t, y <= Omega (robust.c:139-141)."""
+    exps = range(-320, 309, 23)
+    for et in exps:
+        for ex in exps:
+            for ey in (-300, -10, 0, 100, 299):
+                t, x, y = 1.5 * 10.0 ** et, 1.25 * 10.0 ** ex, 10.0 ** ey
+                if not (math.isfinite(t) and math.isfinite(x)) or t > OMEGA:
+                    continue
+                a = oracle.protect_update(t, x, y)
+                assert 0.0 < a <= 1.0
+                # in logs: log(a*y + t*a*x) <= log(Omega) (+ rounding slack)
+                lhs = max(_log2(a) + _log2(y), _log2(a) + _log2(t) + _log2(x))
+                assert lhs <= math.log2(OMEGA) + 1e-9
+                safe = (_log2(t) + _log2(x) < math.log2(OMEGA - y) - 1e-6)
+                if safe:
+                    assert a == 1.0
+
+
+def test_solve_1x1_known_answers():
+    # plain division
+    x, sc, info = oracle.solve_1x1(1e-300, 3.0, 1.0, 4.0)
+    assert (x, sc, info) == (2.0, 1.0, 0)
+    # tiny pivot raised to smin, flagged
+    x, sc, info = oracle.solve_1x1(1e-10, 1.0, 1.0, 1.0)
+    assert info == 1 and x == 1.0 / 1e-10
+    # huge quotient: scaled so the division cannot overflow, x/scale exact
+    x, sc, info = oracle.solve_1x1(1e-320, 1e-200, 0.0, 1e200)
+    assert math.isfinite(x) and 0 < sc < 1 and abs(x) <= 1e300 * 1.0000001
+    assert math.isclose(math.log10(abs(x)) - math.log10(sc), 400.0, rel_tol=1e-12)
+    # sum of same-sign huge terms: halved
+    x, sc, info = oracle.solve_1x1(1e-300, 8e299, -5e299, 1.0)
+    assert sc == 2.0 and math.isclose(x, 2.0 / 1.3e300, rel_tol=1e-15)
+
+
+def test_nrm2_blue_vs_numpy():
+    rng = np.random.default_rng(1)
+    for scale in (1e-300, 1e-170, 1.0, 1e160, 1e300):
+        v = rng.standard_normal(1000) * scale
+        ref = np.linalg.norm(v / scale) * scale
+        assert math.isclose(oracle.nrm2_blue(v), ref, rel_tol=1e-14)
+    mixed = np.array([1e-200, 3.0, 1e200, -4.0])
+    assert math.isclose(oracle.nrm2_blue(mixed), 1e200, rel_tol=1e-15)
+
+
+def test_partition_first_tile_ragged():
+    assert oracle.partition(10, 3, 4) == [0, 2, 6, 10]
+    assert oracle.partition(8, 2, 4) == [0, 4, 8]
+    assert oracle.partition(3, 1, 4) == [0, 3]
+
+
+def test_dlange_inf():
+    A = np.asfortranarray(np.arange(12.0).reshape(3, 4) - 5)
+    assert oracle.dlange_inf(A) == np.abs(A).sum(axis=1).max()
+
+
+# ------------------------------------------------------------- whole path
+
+@pytest.mark.parametrize("n,tlsz,rt", [(64, 64, 32), (200, 64, 8), (201, 32, 5), (300, 8, 3),
+                                       (333, 128, 16), (500, 4, 32), (1000, 64, 32)])
+def test_matches_dense_solve(n, tlsz, rt):
+    """One inverse-iteration step is x = (H - s I)^-1 (eps3 * 1), normalised:
+    dense LU (numpy/LAPACK dgesv) fixes direction AND sign."""
+    H, wr = case(n, seed=n + tlsz, m=12)
+    x, wo, st = oracle.tiled_inverse_iteration(H, wr, tlsz, rt)
+    assert st.nonconverged == 0 and np.array_equal(wo, wr)
+    ref = np.column_stack([dense_direction(H, s) for s in wr])
+    assert (1.0 - cosines(x, ref)).max() <= 1e-10
+
+
+def test_closed_form_n1_n2():
+    x, wo, st = oracle.tiled_inverse_iteration(np.array([[3.0]]), np.array([3.0 * (1 + 4e-16)]))
+    assert st.nonconverged == 0 and x[0, 0] == -1.0   # (h - s) < 0, b > 0
+    a, b, c, d = 2.0, 1.0, 0.5, 1.0
+    H = np.array([[a, b], [c, d]], order="F")
+    lam = 0.5 * (a + d) + math.sqrt(0.25 * (a - d) ** 2 + b * c)
+    s = np.nextafter(lam, np.inf)          # growth > 0.1/sqrt(2) needs |lam - s| ~ ulp
+    x, _, st = oracle.tiled_inverse_iteration(H, np.array([s]))
+    assert st.nonconverged == 0
+    ref = np.array([(d - s) - b, -c + (a - s)])   # adj(H - sI) 1; sign of det unreliable
+    ref /= np.linalg.norm(ref)
+    assert 1 - abs(ref @ x[:, 0]) <= 1e-12
+
+
+@pytest.mark.parametrize("tlsz", [4, 16, 64])
+def test_triangular_reduces_to_back_substitution(tlsz):
+    """Zero subdiagonal: every rotation is the identity, the path is a robust
+    back substitution of (H - s I) -- a textbook dtrsv on unscaled data."""
+    rng = np.random.default_rng(4)
+    n = 150
+    H = np.asfortranarray(np.triu(rng.standard_normal((n, n))))
+    wr = np.diag(H)[:8] * (1 + 1e-12)
+    x, wo, st = oracle.tiled_inverse_iteration(H, wr, tlsz, 4)
+    assert st.nonconverged == 0
+    for k, s in enumerate(wr):
+        ref = sla.solve_triangular(H - s * np.eye(n), eps3_of(H) * np.ones(n))
+        ref /= np.linalg.norm(ref)
+        assert 1 - ref @ x[:, k] <= 1e-12
+
+
+def test_graded_exact_similarity_mild():
+    """H = D H0 D^-1 (exact in FP64) has eigenvectors D x0; with a mild
+    grading the comparison D^-1 x vs x0 is well conditioned."""
+    n = 300
+    H0, wr = case(n, seed=31, m=10)
+    k = w.grading_exponents(n, 3, 31)
+    G = w.graded(H0, 3, 31)
+    x0, _, _ = oracle.tiled_inverse_iteration(H0, wr, 64, 8)
+    xg, _, st = oracle.tiled_inverse_iteration(G, wr, 64, 8)
+    assert st.nonconverged == 0
+    y = np.ldexp(xg, -k[:, None].astype(np.int32))
+    assert one_minus_abs_cos(y, x0).max() <= 1e-10
+
+
+@pytest.mark.parametrize("n,kmax", [(200, 400), (1000, 400)])
+def test_graded_overflow_provoking(n, kmax):
+    """k in [-400, 400]: entries up to 2^800, the solve must scale: finite,
+    converged, residual <= 10 n, and the protections did trigger."""
+    H, wr = case(n, seed=n, m=16, kmax=kmax)
+    x, wo, st = oracle.tiled_inverse_iteration(H, wr, 64, 8)
+    assert np.isfinite(x).all() and st.nonconverged == 0
+    assert st.scalings > 0
+    ns, _ = oracle.residuals(H, wo, x)
+    assert ns.max() <= 10 * n
+
+
+def test_restart_stable_partition():
+    """Shifts half way between eigenvalues cannot converge: they are moved
+    behind the converged ones (stable), restarted with dlaein vectors and
+    left unconverged after n tries (FIX of inverse_iteration.c:75-125)."""
+    H, wr = case(12, seed=3, pert=4e-16)
+    wr = np.sort(wr)
+    mixed = np.array([0.5 * (wr[0] + wr[1]), wr[0], 0.5 * (wr[1] + wr[2]), wr[1]])
+    x, wo, st = oracle.tiled_inverse_iteration(H, mixed, 4, 2)
+    assert st.nonconverged == 2
+    assert st.restarts == 12
+    assert list(wo) == [mixed[1], mixed[3], mixed[0], mixed[2]]
+    assert np.isfinite(x).all()
+
+
+def test_residual_definition():
+    """Both residual forms on arbitrary vectors (no cancellation noise)."""
+    H, wr = case(120, seed=8, m=6)
+    rng = np.random.default_rng(2)
+    x = np.asfortranarray(rng.standard_normal((120, 6)))
+    ns, ref = oracle.residuals(H, wr, x)
+    hinf = np.abs(H).sum(axis=1).max()
+    hf = np.linalg.norm(np.triu(H, -1))
+    for k in range(len(wr)):
+        r = H @ x[:, k] - wr[k] * x[:, k]
+        nx = np.linalg.norm(x[:, k])
+        assert math.isclose(ns[k], np.linalg.norm(r) / (hinf * nx * 2.0 ** -52), rel_tol=1e-12)
+        assert math.isclose(ref[k], np.linalg.norm(r) / (hf * nx + abs(wr[k]) * nx), rel_tol=1e-12)
+    xe, wo, _ = oracle.tiled_inverse_iteration(H, wr, 64, 8)
+    assert oracle.residuals(H, wo, xe)[0].max() <= 10 * 120
+
+
+def test_constructed_spectrum_known_eigenvalues():
+    lam, shifts = w.cluster_spectrum(400, 30, 10, 2)
+    H = w.constructed_spectrum(lam, 2)
+    x, wo, st = oracle.tiled_inverse_iteration(H, shifts, 64, 8)
+    assert st.nonconverged == 0
+    ns, _ = oracle.residuals(H, wo, x)
+    assert ns.max() <= 10 * 400

@@ -17,7 +17,8 @@
 #include "hsrq_internal.h"
 
 namespace hsrq {
-hipError_t launch_norms(const ProbDev*, const RowItem*, int, hipStream_t);
+hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, int, hipStream_t);
+bool norm_block_needed_host(const ProbDev& P, int rb, int cb);
 hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
@@ -84,6 +85,8 @@ struct hsrq_plan {
   // item tables (device offsets)
   const RowItem* d_norm_items = nullptr;
   int n_norm_items = 0;
+  const GemmItem* d_part_items = nullptr;
+  int n_part_items = 0;
   const RowItem* d_init_items = nullptr;
   int n_init_items = 0;
   const ShiftItem* d_bt_items = nullptr;
@@ -138,7 +141,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   p->probs.resize(count);
   p->status_off.resize(count + 1, 0);
   struct Lay {
-    size_t rt0, rt1, c, s, bm, aux, ynb, rowinf, diagn, scal, e, sy, sz, st;
+    size_t rt0, rt1, c, s, bm, aux, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
   };
   std::vector<Lay> lay(count);
   for (int i = 0; i < count; ++i) {
@@ -171,6 +174,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     L.ynb = bytes; bytes += align_up((size_t)P.nrb * q.m * sizeof(double));
     L.rowinf = bytes; bytes += align_up((size_t)P.T * sizeof(double));
     L.diagn = bytes; bytes += align_up((size_t)P.T * sizeof(double));
+    L.part = bytes; bytes += align_up((size_t)P.nrb * q.n * sizeof(double));
     L.scal = bytes; bytes += align_up(4 * sizeof(double));
     L.e = bytes; bytes += align_up(Tm * sizeof(int));
     L.sy = bytes; bytes += align_up(Tm * sizeof(int));
@@ -200,6 +204,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     P.Ynb = (double*)(base + L.ynb);
     P.rowinf = (double*)(base + L.rowinf);
     P.diagn = (double*)(base + L.diagn);
+    P.part = (double*)(base + L.part);
     P.scal = (double*)(base + L.scal);
     P.E = (int*)(base + L.e);
     P.SY = (int*)(base + L.sy);
@@ -209,6 +214,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
 
   // --- item tables ---
   std::vector<RowItem> norm_items, init_items;
+  std::vector<GemmItem> part_items;
   std::vector<ShiftItem> bt_items;
   int steps = 0;
   for (int i = 0; i < count; ++i) {
@@ -216,6 +222,9 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     if (P.m == 0) continue;
     steps = std::max(steps, P.T);
     for (int t = 0; t < P.T; ++t) norm_items.push_back({i, t, 0, 0});
+    for (int rb = 0; rb < P.nrb; ++rb)
+      for (int cb = 0; cb < P.nrb; ++cb)
+        if (norm_block_needed_host(P, rb, cb)) part_items.push_back({i, 0, rb, cb});
     for (int rb = 0; rb < P.nrb; ++rb)
       for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init_items.push_back({i, rb, k0, 0});
     for (int k = 0; k < P.m; ++k) bt_items.push_back({i, k, 0, 0});
@@ -240,6 +249,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   }
   size_t nbytes = 0;
   nbytes += align_up(norm_items.size() * sizeof(RowItem));
+  nbytes += align_up(part_items.size() * sizeof(GemmItem));
   nbytes += align_up(init_items.size() * sizeof(RowItem));
   nbytes += align_up(bt_items.size() * sizeof(ShiftItem));
   for (int s = 0; s < steps; ++s) {
@@ -263,6 +273,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   };
   p->d_norm_items = (const RowItem*)put(norm_items.data(), norm_items.size() * sizeof(RowItem));
   p->n_norm_items = (int)norm_items.size();
+  p->d_part_items = (const GemmItem*)put(part_items.data(), part_items.size() * sizeof(GemmItem));
+  p->n_part_items = (int)part_items.size();
   p->d_init_items = (const RowItem*)put(init_items.data(), init_items.size() * sizeof(RowItem));
   p->n_init_items = (int)init_items.size();
   p->d_bt_items = (const ShiftItem*)put(bt_items.data(), bt_items.size() * sizeof(ShiftItem));
@@ -292,7 +304,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
 int plan_enqueue(hsrq_plan* p) {
   hipStream_t st = p->ctx->stream;
   const int rpl = p->b <= 64 ? 1 : (p->b <= 128 ? 2 : 4);
-  HIP_TRY(launch_norms(p->d_probs, p->d_norm_items, p->n_norm_items, st));
+  HIP_TRY(launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
+                                  p->n_norm_items, st));
   HIP_TRY(launch_init(p->d_probs, p->d_init_items, p->n_init_items, st));
   for (int s = 0; s < p->steps; ++s) {
     HIP_TRY(launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], st));
@@ -320,7 +333,8 @@ int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
     kind.push_back(k);
     return HSRQ_OK;
   };
-  int rc = rec(0, [&] { return launch_norms(p->d_probs, p->d_norm_items, p->n_norm_items, st); });
+  int rc = rec(0, [&] { return launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
+                                  p->n_norm_items, st); });
   if (!rc) rc = rec(1, [&] { return launch_init(p->d_probs, p->d_init_items, p->n_init_items, st); });
   for (int s = 0; s < p->steps && !rc; ++s) {
     rc = rec(2, [&] { return launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], st); });

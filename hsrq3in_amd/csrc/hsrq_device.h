@@ -51,10 +51,15 @@ __device__ __forceinline__ double protect_sum(double x, double y) {
 }
 
 // robust.c:244-262: alpha such that (alpha*y) - t*(alpha*x) cannot overflow.
+// When t and x are both below 2^500 their product is exact enough to decide
+// without the reference's division (the two tests differ only within a
+// rounding of the threshold Omega - y, where either answer is safe).
 __device__ __forceinline__ double protect_update(double tnorm, double xnorm, double ynorm) {
   double scale = 1.0;
   if (xnorm <= 1.0) {
     if (tnorm * xnorm > kOmega - ynorm) scale = 0.5;
+  } else if (tnorm < 0x1p500 && xnorm < 0x1p500) {
+    if (tnorm * xnorm > kOmega - ynorm) scale = 0.5 / xnorm;
   } else {
     if (tnorm > (kOmega - ynorm) / xnorm) scale = 0.5 / xnorm;
   }

@@ -36,6 +36,7 @@ struct ProbDev {
   double* Ynb;        // per row block max |y|, nrb x m
   double* rowinf;     // ||H(I,:)||_oo per tile row, T
   double* diagn;      // ||H(J,J)||_oo per diagonal tile, T
+  double* part;       // partial absolute row sums, nrb x n
   double* scal;       // [0] = ||H||_oo, [1] = eps3
   int* status;        // per shift: bit0 converged, bit1 perturbed pivot
   int its;            // -1: start vector eps3*1; >= 0: dlaein restart vector

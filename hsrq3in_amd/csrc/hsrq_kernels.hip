@@ -26,13 +26,54 @@ namespace hsrq {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// K0: tile norms.  rowinf[I] = ||H(I, :)||_oo (all n columns, as the
-// reference's dlange('I', n, n, H) of inverse_iteration.c:154), diagn[I] =
-// ||H(I, I)||_oo of the b x b diagonal tile (solve-task.c:30-34).
-// One workgroup per (problem, tile); one thread per row.
+// K0: norms of H.  rowinf[I] = ||H(I, :)||_oo (the reference's
+// dlange('I', n, n, H), inverse_iteration.c:154, taken per tile row) and
+// diagn[I] = ||H(I, I)||_oo of the diagonal tile (solve-task.c:30-34), over
+// the Hessenberg part (j >= i - 1; entries below are zero by contract).
+// Pass 1: absolute row sums of 64 x 64 blocks (tile-aligned like the row
+// blocks) into part[cb * n + i]; pass 2: fixed-order sums and maxima per
+// tile (deterministic, no atomics).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_norms(const ProbDev* __restrict__ probs,
-                                               const RowItem* __restrict__ items) {
+__host__ __device__ inline bool norm_block_needed(const ProbDev& P, int rb, int cb) {
+  int t1, r0, nr, t2, c0, nc;
+  rowblock_range(P, rb, t1, r0, nr);
+  rowblock_range(P, cb, t2, c0, nc);
+  return c0 + nc - 1 >= r0 - 1;
+}
+
+__global__ __launch_bounds__(256) void k_norm_part(const ProbDev* __restrict__ probs,
+                                                   const GemmItem* __restrict__ items) {
+  __shared__ double red[4][64];
+  const GemmItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  int t1, r0, nr, t2, c0, nc;
+  rowblock_range(P, it.rb, t1, r0, nr);
+  rowblock_range(P, it.cb, t2, c0, nc);
+  const int row = threadIdx.x & 63, g = threadIdx.x >> 6;
+  double a = 0.0;
+  if (row < nr) {
+    const int i = r0 + row;
+    const double* h = P.H + i;
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int cc = g + 4 * q;
+      const int j = c0 + cc;
+      v[q] = (cc < nc && j >= i - 1) ? fabs(h[(int64_t)j * P.ldH]) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a += v[q];
+  }
+  red[g][row] = a;
+  __syncthreads();
+  if (threadIdx.x < nr)
+    P.part[(int64_t)it.cb * P.n + r0 + threadIdx.x] =
+        (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_norm_final(const ProbDev* __restrict__ probs,
+                                                    const RowItem* __restrict__ items) {
+  __shared__ double red[2][4];
   const RowItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const int I = it.rb;
@@ -40,21 +81,15 @@ __global__ __launch_bounds__(256) void k_norms(const ProbDev* __restrict__ probs
   const int r = threadIdx.x;
   double rs = 0.0, ds = 0.0;
   if (r < nb) {
-    const double* h = P.H + l + r;
-    const int64_t ld = P.ldH;
-    int j = 0;
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    for (; j + 4 <= P.n; j += 4) {
-      a0 += fabs(h[(j + 0) * ld]);
-      a1 += fabs(h[(j + 1) * ld]);
-      a2 += fabs(h[(j + 2) * ld]);
-      a3 += fabs(h[(j + 3) * ld]);
+    const int rb = tile_first_rowblock(P, I) + r / kRowBlock;
+    const int cbd0 = tile_first_rowblock(P, I), cbd1 = cbd0 + tile_num_rowblocks(P, I);
+    for (int cb = 0; cb < P.nrb; ++cb) {
+      if (!norm_block_needed(P, rb, cb)) continue;
+      const double v = P.part[(int64_t)cb * P.n + l + r];
+      rs += v;
+      if (cb >= cbd0 && cb < cbd1) ds += v;
     }
-    for (; j < P.n; ++j) a0 += fabs(h[j * ld]);
-    rs = (a0 + a1) + (a2 + a3);
-    for (int c = 0; c < nb; ++c) ds += fabs(h[(int64_t)(l + c) * ld]);
   }
-  __shared__ double red[2][4];
   const int wave = threadIdx.x >> 6;
   const double mr = wave_max(rs), md = wave_max(ds);
   if ((threadIdx.x & 63) == 0) {
@@ -130,7 +165,7 @@ template <int RPL>
 __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
                                               const K1Item* __restrict__ items) {
   constexpr int BMAX = 64 * RPL;
-  constexpr int CW = (RPL == 4) ? 32 : 64;   // tile columns per LDS chunk
+  constexpr int CW = (RPL == 4) ? 16 : 64;   // tile columns per LDS chunk
   __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
 
   const K1Item it = items[blockIdx.x];
@@ -166,7 +201,35 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   const double slack = 1.0 + 0x1p-40;
   const bool exact = P.force_exact != 0;
 
-  // Upper bounds of |y| and |v| over the rows still active (<= j).
+  // H tile chunks: CW columns each, staged through registers one chunk ahead
+  // so the global loads overlap the sweep of the previous chunk.
+  constexpr int NL = CW * BMAX / 256;
+  double pf[NL];
+  auto issue = [&](int chunk) {
+    const int c0 = chunk * CW;
+    const int nc = min(CW, nb - c0);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int c = idx / BMAX, r = idx - c * BMAX;
+      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+    }
+  };
+  int cur = -1;
+  auto column = [&](int col) -> const double* {
+    const int ch = col / CW;
+    if (ch != cur) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < NL; ++i) Hs[threadIdx.x + 256 * i] = pf[i];
+      __syncthreads();
+      if (ch > 0) issue(ch - 1);
+      cur = ch;
+    }
+    return &Hs[(col - ch * CW) * BMAX];
+  };
+
+  // Upper bounds of |y| and |v| over the rows still active.
   double By, Bv;
   {
     double ly = 0.0, lv = 0.0;
@@ -179,127 +242,171 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     Bv = wave_max(lv);
   }
 
-  int cur = -1;
-  for (int j = nb - 1; j >= 1; --j) {
-    const int col = j - 1;
-    const int ch = col / CW;
-    if (ch != cur) {
-      __syncthreads();
-      const int c0 = ch * CW;
-      const int nc = min(CW, nb - c0);
-      for (int idx = threadIdx.x; idx < nc * BMAX; idx += 256) {
-        const int c = idx / BMAX, r = idx - c * BMAX;
-        Hs[c * BMAX + r] = (r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
-      }
-      __syncthreads();
-      cur = ch;
-    }
-    const double* hc = &Hs[(col - ch * CW) * BMAX];
-    const int jl = j & 63, jq = j >> 6;
-
-    // Rotation G_j from the running column (reduce.c:45 / solve-task.c:171).
-    const double vj = readlane(pick<RPL>(v, jq), jl);
-    const double hsub = hc[j];
+  // ---- v-chain step st: rotation G_st from the running column
+  // (reduce.c:45-51 / solve-task.c:171), column st of R for rows < st
+  // (r_i = c v_i - s h_i, h_{st-1} = H(st-1,st-1) - mu) and the next running
+  // column (v_i = c h_i + s v_i).  Fast form: LAPACK's unscaled dlartg branch;
+  // `rare` flags every input the fast form does not cover.
+  struct VOut {
+    double c, s, t, lam, br, bv;
+  };
+  auto vstep = [&](int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
+                   double (&nv)[RPL]) -> bool {
+    const double vs = readlane(pick<RPL>(v, st >> 6), st & 63);
+    const double hsub = hc[st];
+    const double hdd = hc[st - 1] - mu;
     double c, s;
-    givens(vj, hsub, c, s);
-    if (lane == jl) {
-#pragma unroll
-      for (int q = 0; q < RPL; ++q)
-        if (q == jq) {
-          cr[q] = c;
-          sr[q] = s;
-        }
+    bool rare = false;
+    if (full) {
+      givens(vs, hsub, c, s);
+    } else {
+      const double f1 = fabs(vs), g1 = fabs(hsub);
+      rare = !(f1 > 0x1p-511 && f1 < 0x1.6a09e667f3bcdp+510 && g1 > 0x1p-511 &&
+               g1 < 0x1.6a09e667f3bcdp+510);
+      const double d = sqrt(vs * vs + hsub * hsub);
+      c = f1 / d;
+      const double r = copysign(d, vs);
+      s = -(hsub / r);
     }
-
-    // y(j) = y(j) / (c v(j) - s H(j,j-1))  (solve-task.c:69-75).
-    double yj = readlane(pick<RPL>(y, jq), jl);
-    const int e1 = solve_1x1(smin, c * vj, s * hsub, yj, info);
-    if (e1 != 0) {
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e1);
-      E += e1;
-      By = ldexp(By, e1);
-    }
-    if (lane == jl) {
-#pragma unroll
-      for (int q = 0; q < RPL; ++q)
-        if (q == jq) y[q] = yj;
-    }
-
-    // Protection of y(j-1) -= ... (solve-task.c:78-82), bounds first.
-    const double hdd = hc[j - 1] - mu;
-    double p1 = exact ? 0.0 : protect_double_update(Bv, fabs(yj), fabs(hdd), fabs(yj), By);
-    double yjm1 = 0.0, vjm1 = 0.0;
-    bool have_jm1 = false;
-    if (p1 != 1.0) {
-      const int il = (j - 1) & 63, iq = (j - 1) >> 6;
-      vjm1 = fabs(readlane(pick<RPL>(v, iq), il));
-      yjm1 = fabs(readlane(pick<RPL>(y, iq), il));
-      have_jm1 = true;
-      p1 = protect_double_update(vjm1, fabs(yj), fabs(hdd), fabs(yj), yjm1);
-    }
-    const int e2 = pow2_exp(p1);
-    const double yj2 = ldexp(yj, e2);
-
-    // Protection of y(0:j-2) -= ... (solve-task.c:84-89), bounds first.
-    double p2 = exact ? 0.0
-                      : protect_double_update(Bv, fabs(yj2), Hnorm, fabs(yj2), ldexp(By, e2));
-    double ynorm_x = -1.0, vnorm_x = -1.0;
-    if (p2 != 1.0) {
-      double ly = 0.0, lv = 0.0;
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        const int r = lane + 64 * q;
-        if (r < j - 1) {
-          ly = fmax(ly, fabs(y[q]));
-          lv = fmax(lv, fabs(v[q]));
-        }
-      }
-      ynorm_x = wave_max(ly);
-      vnorm_x = wave_max(lv);
-      p2 = protect_double_update(vnorm_x, fabs(yj2), Hnorm, fabs(yj2), ldexp(ynorm_x, e2));
-    }
-    const int e3 = pow2_exp(p2);
-    const int et = e2 + e3;
-    if (et != 0) {
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], et);
-      E += et;
-      yj = ldexp(yj, et);
-    }
-
-    // Updates of y(0:j-1) and of the annihilation column v(0:j-1)
-    // (solve-task.c:82, :91-93, :98-101).
-    const double yc = yj * c, ys = yj * s;
+    const double Hm = fmax(Hnorm, fabs(hdd));
+    o.c = c;
+    o.s = s;
+    o.t = c * vs;
+    o.lam = s * hsub;
+    o.br = (fabs(c) * Bv + fabs(s) * Hm) * slack;
+    o.bv = (fabs(c) * Hm + fabs(s) * Bv) * slack;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
-      const int r = lane + 64 * q;
-      if (r <= j - 1) {
-        const double h = (r == j - 1) ? hdd : hc[r];
-        y[q] = y[q] - yc * v[q] + ys * h;
-        v[q] = c * h + s * v[q];
-      }
+      const int row = lane + 64 * q;
+      const bool act = row <= st - 1;
+      const double h = (row == st - 1) ? hdd : hc[row];
+      nr[q] = act ? c * v[q] - s * h : 0.0;
+      nv[q] = act ? c * h + s * v[q] : v[q];
     }
+    return rare;
+  };
 
-    // Bounds for the rows that stay active (<= j-2 after this step): exact
-    // when the exact path was taken, else propagated through the update.
-    if (have_jm1 || ynorm_x >= 0.0) {
-      double ly = 0.0, lv = 0.0;
+  // ---- y-chain step st, fast form: y(st) /= R(st,st) and
+  // y(0:st-1) -= y(st) R(0:st-1, st) when no protection can trigger
+  // (protect_sum = 1, |R(st,st)| >= smin, the division and the update
+  // provably below Omega from the running bounds).
+  auto ystep_fast = [&](int st, const VOut& o, const double (&r)[RPL], double (&ny)[RPL],
+                        double& byn) -> bool {
+    const double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
+    const double nl = -o.lam;
+    bool rare = ((o.t > 0 && nl > 0) || (o.t < 0 && nl < 0)) && fabs(o.t) > kOmega - fabs(nl);
+    const double csr = o.t - o.lam;
+    const double acsr = fabs(csr);
+    rare = rare || !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
+    const double yq = yj / csr;
+    byn = (By + fabs(yq) * o.br) * slack;
+    rare = rare || !(byn <= kOmega);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = lane + 64 * q;
+      ny[q] = (row <= st - 1) ? y[q] - yq * r[q] : ((row == st) ? yq : y[q]);
+    }
+    return rare;
+  };
+
+  // ---- y-chain step st, protected form.  solve_1x1_real_system
+  // (robust.c:453-477) with power-of-two factors, then ONE protection of the
+  // whole update: with a = max_i (|y_i| + |y(st)| |R(i,st)|) (exact, one
+  // wavefront reduction, computed in a scaled form), scale the tile by a power
+  // of two below 1/a whenever a > Omega.
+  auto ystep_slow = [&](int st, const VOut& o, const double (&r)[RPL]) {
+    double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
+    const int e1 = solve_1x1(smin, o.t, o.lam, yj, info);
+    if (e1 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (lane + 64 * q != st) y[q] = ldexp(y[q], e1);
+      E += e1;
+    }
+    const double ay = fabs(yj);
+    const int kk = (ay > 0.0) ? ilogb(ay) : 0;
+    const double ays = ldexp(ay, -kk);
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q)
+      if (lane + 64 * q <= st - 1) a = fmax(a, ldexp(fabs(y[q]), -kk) + ays * fabs(r[q]));
+    a = wave_max(a);
+    int e2 = 0;
+    if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
+    if (e2 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e2);
+      yj = ldexp(yj, e2);
+      E += e2;
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = lane + 64 * q;
+      if (row <= st - 1) y[q] = y[q] - yj * r[q];
+      if (row == st) y[q] = yj;
+    }
+    By = ldexp(a, kk + e2) * slack;
+  };
+
+  auto keep_rotation = [&](int st, const VOut& o) {
+    if (lane == (st & 63)) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (q == (st >> 6)) {
+          cr[q] = o.c;
+          sr[q] = o.s;
+        }
+    }
+  };
+
+  if (nb >= 2) {
+    issue((nb - 2) / CW);
+    VOut cv;
+    double rr[RPL], nr[RPL], nv[RPL], ny[RPL];
+    {
+      const double* hc = column(nb - 2);
+      if (vstep(nb - 1, hc, false, cv, nr, nv)) vstep(nb - 1, hc, true, cv, nr, nv);
 #pragma unroll
       for (int q = 0; q < RPL; ++q) {
-        const int r = lane + 64 * q;
-        if (r <= j - 2) {
-          ly = fmax(ly, fabs(y[q]));
-          lv = fmax(lv, fabs(v[q]));
-        }
+        v[q] = nv[q];
+        rr[q] = nr[q];
       }
-      By = wave_max(ly);
-      Bv = wave_max(lv);
-    } else {
-      const double Hm = fmax(Hnorm, fabs(hdd));
-      const double Byp = ldexp(By, et);
-      By = (Byp + fabs(yj) * (fabs(c) * Bv + fabs(s) * Hm)) * slack;
-      Bv = (fabs(c) * Hm + fabs(s) * Bv) * slack;
+      Bv = cv.bv;
+      keep_rotation(nb - 1, cv);
+    }
+    // Steady state: v-step s-1 and y-step s are independent and interleave.
+    for (int s = nb - 1; s >= 2; --s) {
+      const double* hc = column(s - 2);
+      VOut nvo;
+      double byn;
+      const bool rv = vstep(s - 1, hc, false, nvo, nr, nv);
+      const bool ry = ystep_fast(s, cv, rr, ny, byn);
+      if (rv) vstep(s - 1, hc, true, nvo, nr, nv);
+      if (ry || exact) {
+        ystep_slow(s, cv, rr);
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        By = byn;
+      }
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        v[q] = nv[q];
+        rr[q] = nr[q];
+      }
+      Bv = nvo.bv;
+      cv = nvo;
+      keep_rotation(s - 1, cv);
+    }
+    {
+      double byn;
+      if (ystep_fast(1, cv, rr, ny, byn) || exact) {
+        ystep_slow(1, cv, rr);
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        By = byn;
+      }
     }
   }
 
@@ -783,11 +890,15 @@ __global__ void k_mfma_selftest(int* __restrict__ bad) {
 // ---------------------------------------------------------------------------
 namespace hsrq {
 
-hipError_t launch_norms(const ProbDev* d_probs, const RowItem* items, int count, hipStream_t st) {
+hipError_t launch_norms(const ProbDev* d_probs, const GemmItem* part_items, int part_count,
+                        const RowItem* items, int count, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_norms, dim3(count), dim3(256), 0, st, d_probs, items);
+  hipLaunchKernelGGL(k_norm_part, dim3(part_count), dim3(256), 0, st, d_probs, part_items);
+  hipLaunchKernelGGL(k_norm_final, dim3(count), dim3(256), 0, st, d_probs, items);
   return hipGetLastError();
 }
+
+bool norm_block_needed_host(const ProbDev& P, int rb, int cb) { return norm_block_needed(P, rb, cb); }
 
 hipError_t launch_init(const ProbDev* d_probs, const RowItem* items, int count, hipStream_t st) {
   if (count <= 0) return hipSuccess;

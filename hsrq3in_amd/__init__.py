@@ -50,7 +50,8 @@ class Problem(ctypes.Structure):
 EXPORTS = [
     "tiled_inverse_iteration",
     "hsrq_context_create", "hsrq_context_destroy", "hsrq_context_set_tile",
-    "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_strerror",
+    "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_context_set_lookahead",
+    "hsrq_strerror",
     "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
     "hsrq_plan_sync", "hsrq_plan_status",
@@ -79,6 +80,7 @@ def lib():
         L.hsrq_context_set_tile.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_exact_norms.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_precision.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_set_lookahead.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_stream.restype = _vp
         L.hsrq_context_stream.argtypes = [_vp]
         L.hsrq_selftest.argtypes = [_vp]
@@ -181,6 +183,9 @@ class Context:
     def set_exact_norms(self, on: bool):
         _check(lib().hsrq_context_set_exact_norms(self._h, int(on)), "set_exact_norms")
 
+    def set_lookahead(self, on: bool):
+        _check(lib().hsrq_context_set_lookahead(self._h, int(on)), "set_lookahead")
+
     def set_precision(self, prec: int):
         _check(lib().hsrq_context_set_precision(self._h, prec), "set_precision")
 
@@ -253,12 +258,12 @@ class Plan:
         f = lib().hsrq_plan_execute_graph if graph else lib().hsrq_plan_execute
         _check(f(self._h), "hsrq_plan_execute")
 
-    KERNELS = ("norms", "init", "diag", "update", "backtransform")
+    KERNELS = ("norms", "init", "diag", "update", "backtransform", "params")
 
     def execute_profiled(self) -> dict:
         """Synchronous run with HIP events per launch: {kernel: (ms, launches)}."""
-        ms = (ctypes.c_double * 5)()
-        cnt = (ctypes.c_int * 5)()
+        ms = (ctypes.c_double * 6)()
+        cnt = (ctypes.c_int * 6)()
         _check(lib().hsrq_plan_execute_profiled(self._h, ms, cnt), "hsrq_plan_execute_profiled")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNELS)}
 

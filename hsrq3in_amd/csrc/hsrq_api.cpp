@@ -22,6 +22,7 @@ bool norm_block_needed_host(const ProbDev& P, int rb, int cb);
 hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
+hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);
 hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
@@ -51,6 +52,7 @@ struct hsrq_context {
   int tile = 0;          // 0: choose per problem
   int exact = 0;
   int precision = HSRQ_PREC_FP64;
+  int lookahead = 0;     // 1: bulk panel pass on a second stream
 };
 
 namespace {
@@ -93,8 +95,14 @@ struct hsrq_plan {
   int n_bt_items = 0;
   std::vector<const K1Item*> k1_ptr;
   std::vector<int> k1_cnt;
-  std::vector<const GemmItem*> gm_ptr;
-  std::vector<int> gm_cnt;
+  // panel passes: rows of the next diagonal tile ("a", on the main stream,
+  // it gates the next diagonal sweep) and all rows above it ("b", on the
+  // side stream, overlapping the next diagonal sweep)
+  std::vector<const GemmItem*> ga_ptr, gb_ptr;
+  std::vector<int> ga_cnt, gb_cnt;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  std::vector<hipEvent_t> ev_p, ev_b;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
@@ -105,6 +113,11 @@ void plan_free(hsrq_plan* p) {
   if (!p) return;
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (auto e : p->ev_p) (void)hipEventDestroy(e);
+  for (auto e : p->ev_b) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->d_work) (void)hipFree(p->d_work);
   if (p->d_probs) (void)hipFree(p->d_probs);
   if (p->d_items) (void)hipFree(p->d_items);
@@ -141,7 +154,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   p->probs.resize(count);
   p->status_off.resize(count + 1, 0);
   struct Lay {
-    size_t rt0, rt1, c, s, bm, aux, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
+    size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
   };
   std::vector<Lay> lay(count);
   for (int i = 0; i < count; ++i) {
@@ -169,8 +182,10 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     L.rt1 = bytes; bytes += align_up(nm);
     L.c = bytes; bytes += align_up(nm);
     L.s = bytes; bytes += align_up(nm);
-    L.bm = bytes; bytes += align_up((size_t)b * 2 * q.m * sizeof(double));
-    L.aux = bytes; bytes += align_up((size_t)2 * q.m * sizeof(double));
+    L.bm0 = bytes; bytes += align_up((size_t)b * 2 * q.m * sizeof(double));
+    L.bm1 = bytes; bytes += align_up((size_t)b * 2 * q.m * sizeof(double));
+    L.aux0 = bytes; bytes += align_up((size_t)4 * q.m * sizeof(double));
+    L.aux1 = bytes; bytes += align_up((size_t)4 * q.m * sizeof(double));
     L.ynb = bytes; bytes += align_up((size_t)P.nrb * q.m * sizeof(double));
     L.rowinf = bytes; bytes += align_up((size_t)P.T * sizeof(double));
     L.diagn = bytes; bytes += align_up((size_t)P.T * sizeof(double));
@@ -199,8 +214,10 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     P.Rt[1] = (double*)(base + L.rt1);
     P.C = (double*)(base + L.c);
     P.S = (double*)(base + L.s);
-    P.Bm = (double*)(base + L.bm);
-    P.aux = (double*)(base + L.aux);
+    P.Bm[0] = (double*)(base + L.bm0);
+    P.Bm[1] = (double*)(base + L.bm1);
+    P.aux[0] = (double*)(base + L.aux0);
+    P.aux[1] = (double*)(base + L.aux1);
     P.Ynb = (double*)(base + L.ynb);
     P.rowinf = (double*)(base + L.rowinf);
     P.diagn = (double*)(base + L.diagn);
@@ -224,14 +241,14 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     for (int t = 0; t < P.T; ++t) norm_items.push_back({i, t, 0, 0});
     for (int rb = 0; rb < P.nrb; ++rb)
       for (int cb = 0; cb < P.nrb; ++cb)
-        if (norm_block_needed_host(P, rb, cb)) part_items.push_back({i, 0, rb, cb});
+        if (norm_block_needed_host(P, rb, cb)) part_items.push_back({i, 0, rb, cb, 1, 0});
     for (int rb = 0; rb < P.nrb; ++rb)
       for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init_items.push_back({i, rb, k0, 0});
     for (int k = 0; k < P.m; ++k) bt_items.push_back({i, k, 0, 0});
   }
   p->steps = steps;
   std::vector<std::vector<K1Item>> k1(steps);
-  std::vector<std::vector<GemmItem>> gm(steps);
+  std::vector<std::vector<GemmItem>> ga(steps), gb(steps);
   for (int s = 0; s < steps; ++s) {
     for (int i = 0; i < count; ++i) {
       const ProbDev& P = p->probs[i];
@@ -240,10 +257,26 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       if (J < 0) continue;
       for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) k1[s].push_back({i, J, k0, 0});
       if (J >= 1) {
+        // rows of the next diagonal tile (part a) and all rows above (part
+        // b), in pairs of row blocks; XCD-aware order: blocks are dealt to
+        // the 8 XCDs round robin, so the pairs 8t..8t+7 go out interleaved
+        // and every XCD sweeps all column blocks of one pair from its L2.
         const int nrbJ = tile_first_rowblock(P, J);
+        const int rbn = tile_first_rowblock(P, J - 1);
         const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
-        for (int rb = 0; rb < nrbJ; ++rb)
-          for (int cb = 0; cb < ncb; ++cb) gm[s].push_back({i, J, rb, cb});
+        auto emit = [&](std::vector<GemmItem>& dst, int rs, int re) {
+          std::vector<std::pair<int, int>> pairs;
+          for (int rb = rs; rb < re; rb += kGemmRowBlocks)
+            pairs.push_back({rb, std::min(kGemmRowBlocks, re - rb)});
+          for (size_t t = 0; t < pairs.size(); t += 8) {
+            const size_t te = std::min(pairs.size(), t + 8);
+            for (int cb = 0; cb < ncb; ++cb)
+              for (size_t q = t; q < te; ++q)
+                dst.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
+          }
+        };
+        emit(ga[s], rbn, nrbJ);
+        emit(gb[s], 0, rbn);
       }
     }
   }
@@ -254,7 +287,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   nbytes += align_up(bt_items.size() * sizeof(ShiftItem));
   for (int s = 0; s < steps; ++s) {
     nbytes += align_up(k1[s].size() * sizeof(K1Item));
-    nbytes += align_up(gm[s].size() * sizeof(GemmItem));
+    nbytes += align_up(ga[s].size() * sizeof(GemmItem));
+    nbytes += align_up(gb[s].size() * sizeof(GemmItem));
   }
   std::vector<char> host(std::max<size_t>(nbytes, kAlign));
   e = hipMalloc(&p->d_items, std::max<size_t>(nbytes, kAlign));
@@ -281,13 +315,17 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   p->n_bt_items = (int)bt_items.size();
   p->k1_ptr.resize(steps);
   p->k1_cnt.resize(steps);
-  p->gm_ptr.resize(steps);
-  p->gm_cnt.resize(steps);
+  p->ga_ptr.resize(steps);
+  p->ga_cnt.resize(steps);
+  p->gb_ptr.resize(steps);
+  p->gb_cnt.resize(steps);
   for (int s = 0; s < steps; ++s) {
     p->k1_ptr[s] = (const K1Item*)put(k1[s].data(), k1[s].size() * sizeof(K1Item));
     p->k1_cnt[s] = (int)k1[s].size();
-    p->gm_ptr[s] = (const GemmItem*)put(gm[s].data(), gm[s].size() * sizeof(GemmItem));
-    p->gm_cnt[s] = (int)gm[s].size();
+    p->ga_ptr[s] = (const GemmItem*)put(ga[s].data(), ga[s].size() * sizeof(GemmItem));
+    p->ga_cnt[s] = (int)ga[s].size();
+    p->gb_ptr[s] = (const GemmItem*)put(gb[s].data(), gb[s].size() * sizeof(GemmItem));
+    p->gb_cnt[s] = (int)gb[s].size();
   }
   e = hipMemcpy(p->d_items, host.data(), nbytes, hipMemcpyHostToDevice);
   if (e == hipSuccess && count > 0)
@@ -297,66 +335,121 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     plan_free(p);
     return HSRQ_EHIP;
   }
+  e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
+  p->ev_p.assign(steps, nullptr);
+  p->ev_b.assign(steps, nullptr);
+  for (int s = 0; s < steps && e == hipSuccess; ++s) {
+    e = hipEventCreateWithFlags(&p->ev_p[s], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_b[s], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    last_hip_error() = e;
+    plan_free(p);
+    return HSRQ_EHIP;
+  }
   *out = p;
   return HSRQ_OK;
 }
 
-int plan_enqueue(hsrq_plan* p) {
-  hipStream_t st = p->ctx->stream;
-  const int rpl = p->b <= 64 ? 1 : (p->b <= 128 ? 2 : 4);
-  HIP_TRY(launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
-                                  p->n_norm_items, st));
-  HIP_TRY(launch_init(p->d_probs, p->d_init_items, p->n_init_items, st));
-  for (int s = 0; s < p->steps; ++s) {
-    HIP_TRY(launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], st));
-    HIP_TRY(launch_update(p->d_probs, p->gm_ptr[s], p->gm_cnt[s], st));
-  }
-  HIP_TRY(launch_backtransform(p->d_probs, p->d_bt_items, p->n_bt_items, st));
-  return HSRQ_OK;
-}
-
-// Same schedule with HIP events around every launch; kernel_ms[kind] gets the
-// summed device time of each kernel kind (norms, init, diag, update,
-// backtransform) and launches[kind] the number of launches.
-int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
-  hipStream_t st = p->ctx->stream;
-  const int rpl = p->b <= 64 ? 1 : (p->b <= 128 ? 2 : 4);
-  const int nev = 2 * (3 + 2 * p->steps);
-  std::vector<hipEvent_t> ev(nev);
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+// Launch schedule.  Main stream: norms, init, then per step s the diagonal
+// sweeps, the scaling exponents (after the previous step's bulk panel pass)
+// and the panel pass of the rows of the next diagonal tile; side stream: the
+// panel pass of all rows above, which overlaps the next diagonal sweep.
+// With `prof`, every launch is bracketed by timing events on its stream and
+// kernel_ms/launches accumulate per kind (0 norms, 1 init, 2 diag,
+// 3 update, 4 back-transform, 5 params).
+struct Prof {
+  std::vector<hipEvent_t> ev;
   std::vector<int> kind;
-  int q = 0;
-  auto rec = [&](int k, auto&& fn) -> int {
-    HIP_TRY(hipEventRecord(ev[q++], st));
-    HIP_TRY(fn());
-    HIP_TRY(hipEventRecord(ev[q++], st));
-    kind.push_back(k);
+};
+
+int plan_enqueue_impl(hsrq_plan* p, Prof* prof) {
+  hipStream_t st = p->ctx->stream;
+  hipStream_t sd = p->ctx->lookahead ? p->side : st;
+  const int rpl = p->b <= 64 ? 1 : (p->b <= 128 ? 2 : 4);
+  auto run = [&](int kind, hipStream_t s, auto&& fn) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      HIP_TRY(hipEventRecord(e0, s));
+    }
+    HIP_TRY(fn(s));
+    if (prof) {
+      HIP_TRY(hipEventRecord(e1, s));
+      prof->ev.push_back(e0);
+      prof->ev.push_back(e1);
+      prof->kind.push_back(kind);
+    }
     return HSRQ_OK;
   };
-  int rc = rec(0, [&] { return launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
-                                  p->n_norm_items, st); });
-  if (!rc) rc = rec(1, [&] { return launch_init(p->d_probs, p->d_init_items, p->n_init_items, st); });
-  for (int s = 0; s < p->steps && !rc; ++s) {
-    rc = rec(2, [&] { return launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], st); });
-    if (!rc && p->gm_cnt[s] > 0)
-      rc = rec(3, [&] { return launch_update(p->d_probs, p->gm_ptr[s], p->gm_cnt[s], st); });
+  int rc;
+  rc = run(0, st, [&](hipStream_t s) {
+    return launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
+                        p->n_norm_items, s);
+  });
+  if (rc) return rc;
+  rc = run(1, st, [&](hipStream_t s) {
+    return launch_init(p->d_probs, p->d_init_items, p->n_init_items, s);
+  });
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(p->ev_fork, st));
+  HIP_TRY(hipStreamWaitEvent(sd, p->ev_fork, 0));
+  int last_b = -1;
+  for (int s = 0; s < p->steps; ++s) {
+    rc = run(2, st, [&](hipStream_t x) {
+      return launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
+    });
+    if (rc) return rc;
+    if (p->ga_cnt[s] + p->gb_cnt[s] == 0) continue;
+    if (last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, p->ev_b[last_b], 0));
+    rc = run(5, st, [&](hipStream_t x) {
+      return launch_params(p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
+    });
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev_p[s], st));
+    rc = run(3, st, [&](hipStream_t x) {
+      return launch_update(p->d_probs, p->ga_ptr[s], p->ga_cnt[s], x);
+    });
+    if (rc) return rc;
+    HIP_TRY(hipStreamWaitEvent(sd, p->ev_p[s], 0));
+    rc = run(3, sd, [&](hipStream_t x) {
+      return launch_update(p->d_probs, p->gb_ptr[s], p->gb_cnt[s], x);
+    });
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(p->ev_b[s], sd));
+    last_b = s;
   }
-  if (!rc)
-    rc = rec(4, [&] { return launch_backtransform(p->d_probs, p->d_bt_items, p->n_bt_items, st); });
+  // join the side stream (also when it had no work: graph capture needs it)
+  HIP_TRY(hipEventRecord(p->ev_join, sd));
+  HIP_TRY(hipStreamWaitEvent(st, p->ev_join, 0));
+  return run(4, st, [&](hipStream_t s) {
+    return launch_backtransform(p->d_probs, p->d_bt_items, p->n_bt_items, s);
+  });
+}
+
+int plan_enqueue(hsrq_plan* p) { return plan_enqueue_impl(p, nullptr); }
+
+int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
+  Prof prof;
+  int rc = plan_enqueue_impl(p, &prof);
   if (!rc) {
-    HIP_TRY(hipStreamSynchronize(st));
-    for (int k = 0; k < 5; ++k) {
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->side));
+    for (int k = 0; k < 6; ++k) {
       kernel_ms[k] = 0.0;
       launches[k] = 0;
     }
-    for (size_t i = 0; i < kind.size(); ++i) {
+    for (size_t i = 0; i < prof.kind.size(); ++i) {
       float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-      kernel_ms[kind[i]] += ms;
-      launches[kind[i]] += 1;
+      HIP_TRY(hipEventElapsedTime(&ms, prof.ev[2 * i], prof.ev[2 * i + 1]));
+      kernel_ms[prof.kind[i]] += ms;
+      launches[prof.kind[i]] += 1;
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto e : prof.ev) (void)hipEventDestroy(e);
   return rc;
 }
 
@@ -486,6 +579,12 @@ int hsrq_context_set_tile(hsrq_context* ctx, int tlsz) {
 int hsrq_context_set_exact_norms(hsrq_context* ctx, int on) {
   if (!ctx) return HSRQ_EINVAL;
   ctx->exact = on ? 1 : 0;
+  return HSRQ_OK;
+}
+
+int hsrq_context_set_lookahead(hsrq_context* ctx, int on) {
+  if (!ctx) return HSRQ_EINVAL;
+  ctx->lookahead = on ? 1 : 0;
   return HSRQ_OK;
 }
 

@@ -118,6 +118,17 @@ __device__ __forceinline__ void givens(double f, double g, double& c, double& s)
   }
 }
 
+// 1/sqrt(x) for x in a safe normal range: hardware estimate refined by two
+// Newton steps y <- y (3 - x y^2) / 2 (relative error ~1 ulp).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  return y;
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront helpers (wave64)
 // ---------------------------------------------------------------------------

@@ -10,8 +10,10 @@ namespace hsrq {
 constexpr int kRowBlock = 64;
 // Shifts per diagonal-sweep workgroup (one wavefront each).
 constexpr int kShiftsPerK1 = 4;
-// Output columns per GEMM block (kGemmCols / 2 shifts: W and Z columns).
+// Output columns per GEMM block (kGemmCols / 2 shifts: W and Z columns) and
+// row blocks per GEMM block.
 constexpr int kGemmCols = 64;
+constexpr int kGemmRowBlocks = 1;
 
 // One problem: H (n x n, column-major, ldH), m real shifts, tile size b.
 // Rows are tiled with the FIRST tile ragged (b0 = n - (T-1) b), as the
@@ -28,8 +30,9 @@ struct ProbDev {
   double* Rt[2];      // running cross-over columns, n x m each (ld n), ping-pong
   double* C;          // cosines, n x m (ld n)
   double* S;          // sines, n x m (ld n)
-  double* Bm;         // GEMM right operand, b x 2m (ld b): col 2k = W_k, 2k+1 = Z_k
-  double* aux;        // 2 per shift: P_k, Zlast_k (prescaled)
+  double* Bm[2];      // GEMM right operand per tile-column parity, b x 2m (ld b):
+                      // col 2k = W_k, 2k+1 = Z_k
+  double* aux[2];     // per parity, 4 per shift: P_k, Zlast_k (prescaled), ||Z_k||, prescale
   int* E;             // scale exponents, T x m  (E[t*m + k])
   int* SY;            // epilogue exponents for y,  T x m
   int* SZ;            // epilogue exponents for the update, T x m
@@ -46,8 +49,8 @@ struct ProbDev {
 struct K1Item {
   int prob, J, k0, pad;
 };
-struct GemmItem {
-  int prob, J, rb, cb;
+struct GemmItem {   // panel pass: row blocks rb .. rb+nrbk-1 (<= 2), column block cb
+  int prob, J, rb, cb, nrbk, pad;
 };
 struct RowItem {  // init: (prob, row block, shift group); K0: (prob, tile)
   int prob, rb, k0, pad;

@@ -245,34 +245,41 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   // ---- v-chain step st: rotation G_st from the running column
   // (reduce.c:45-51 / solve-task.c:171), column st of R for rows < st
   // (r_i = c v_i - s h_i, h_{st-1} = H(st-1,st-1) - mu) and the next running
-  // column (v_i = c h_i + s v_i).  Fast form: LAPACK's unscaled dlartg branch;
-  // `rare` flags every input the fast form does not cover.
+  // column (v_i = c h_i + s v_i).  The pivot R(st,st) = c v_st - s h_sub is
+  // the rotation's r = sign(v_st) ||(v_st, h_sub)|| itself, so it is taken
+  // from the rotation with its reciprocal.  Fast form: reciprocal square root
+  // (in LAPACK 3.10 dlartg's unscaled range); `rare` flags every input
+  // outside it, which the full form (dlartg's branches) handles.
   struct VOut {
-    double c, s, t, lam, br, bv;
+    double c, s, r, rinv, br, bv;
   };
   auto vstep = [&](int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
                    double (&nv)[RPL]) -> bool {
     const double vs = readlane(pick<RPL>(v, st >> 6), st & 63);
     const double hsub = hc[st];
     const double hdd = hc[st - 1] - mu;
-    double c, s;
+    double c, s, r, rinv;
     bool rare = false;
     if (full) {
       givens(vs, hsub, c, s);
+      r = c * vs - s * hsub;
+      rinv = 1.0 / r;
     } else {
       const double f1 = fabs(vs), g1 = fabs(hsub);
       rare = !(f1 > 0x1p-511 && f1 < 0x1.6a09e667f3bcdp+510 && g1 > 0x1p-511 &&
                g1 < 0x1.6a09e667f3bcdp+510);
-      const double d = sqrt(vs * vs + hsub * hsub);
-      c = f1 / d;
-      const double r = copysign(d, vs);
-      s = -(hsub / r);
+      const double d2 = fma(vs, vs, hsub * hsub);
+      const double id = rsqrt_nr(d2);
+      rinv = copysign(id, vs);
+      r = d2 * rinv;
+      c = vs * rinv;
+      s = -(hsub * rinv);
     }
     const double Hm = fmax(Hnorm, fabs(hdd));
     o.c = c;
     o.s = s;
-    o.t = c * vs;
-    o.lam = s * hsub;
+    o.r = r;
+    o.rinv = rinv;
     o.br = (fabs(c) * Bv + fabs(s) * Hm) * slack;
     o.bv = (fabs(c) * Hm + fabs(s) * Bv) * slack;
 #pragma unroll
@@ -288,17 +295,14 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
 
   // ---- y-chain step st, fast form: y(st) /= R(st,st) and
   // y(0:st-1) -= y(st) R(0:st-1, st) when no protection can trigger
-  // (protect_sum = 1, |R(st,st)| >= smin, the division and the update
-  // provably below Omega from the running bounds).
+  // (|R(st,st)| >= smin, the quotient and the update provably below Omega
+  // from the running bounds).
   auto ystep_fast = [&](int st, const VOut& o, const double (&r)[RPL], double (&ny)[RPL],
                         double& byn) -> bool {
     const double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
-    const double nl = -o.lam;
-    bool rare = ((o.t > 0 && nl > 0) || (o.t < 0 && nl < 0)) && fabs(o.t) > kOmega - fabs(nl);
-    const double csr = o.t - o.lam;
-    const double acsr = fabs(csr);
-    rare = rare || !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
-    const double yq = yj / csr;
+    const double acsr = fabs(o.r);
+    bool rare = !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
+    const double yq = yj * o.rinv;
     byn = (By + fabs(yq) * o.br) * slack;
     rare = rare || !(byn <= kOmega);
 #pragma unroll
@@ -310,13 +314,21 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   };
 
   // ---- y-chain step st, protected form.  solve_1x1_real_system
-  // (robust.c:453-477) with power-of-two factors, then ONE protection of the
-  // whole update: with a = max_i (|y_i| + |y(st)| |R(i,st)|) (exact, one
-  // wavefront reduction, computed in a scaled form), scale the tile by a power
-  // of two below 1/a whenever a > Omega.
+  // (robust.c:453-477) on the pivot R(st,st) with power-of-two factors:
+  // |R| < smin -> smin (info), division protected against overflow; then ONE
+  // protection of the whole update: with a = max_i (|y_i| + |y(st)|
+  // |R(i,st)|) (exact, one wavefront reduction, in a scaled form) the tile is
+  // scaled by a power of two below 1/a whenever a > Omega.
   auto ystep_slow = [&](int st, const VOut& o, const double (&r)[RPL]) {
     double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
-    const int e1 = solve_1x1(smin, o.t, o.lam, yj, info);
+    double csr = o.r, cinv = o.rinv;
+    if (fabs(csr) < smin) {
+      csr = smin;
+      cinv = 1.0 / smin;
+      info = 1;
+    }
+    const int e1 = pow2_exp(protect_real_division(yj, csr));
+    yj = ldexp(yj, e1) * cinv;
     if (e1 != 0) {
 #pragma unroll
       for (int q = 0; q < RPL; ++q)
@@ -424,7 +436,13 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       }
     }
     double y0 = readlane(y[0], 0);
-    const int e = solve_1x1(smin, v0, 0.0, y0, info);
+    double csr = v0;
+    if (fabs(csr) < smin) {
+      csr = smin;
+      info = 1;
+    }
+    const int e = pow2_exp(protect_real_division(y0, csr));
+    y0 = ldexp(y0, e) / csr;
     if (e != 0) {
 #pragma unroll
       for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e);
@@ -497,7 +515,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
 
   if (valid) {
-    double* Bw = P.Bm + (int64_t)(2 * k) * P.b;
+    double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
     double* Bz = Bw + P.b;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
@@ -508,29 +526,51 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       }
     }
     if (lane == 0) {
-      P.aux[2 * k] = Pk;
-      P.aux[2 * k + 1] = ldexp(zlast, -p);
+      P.aux[J & 1][4 * k + 0] = Pk;
+      P.aux[J & 1][4 * k + 1] = ldexp(zlast, -p);
+      P.aux[J & 1][4 * k + 2] = zn;
+      P.aux[J & 1][4 * k + 3] = (double)p;
     }
-    // Scaling exponents of the update of every tile row I < J: consolidate
-    // (update-task.c:68-88), then one protection for the whole fused update.
-    // |update(i)| <= (||H(I,:)||_oo + |mu|) ||Z|| + |R~(i)| |Zlast| and
-    // |R~(i)| <= ||(H - mu I)(i,:)||_2, so t_I = 2 (||H(I,:)||_oo + |mu|).
-    const int EJ = E;
-    for (int I = lane; I < J; I += 64) {
-      const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
-      double yn = 0.0;
-      for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, P.Ynb[(int64_t)rb * m + k]);
-      const int EI = P.E[(int64_t)I * m + k];
-      const int c0 = min(EI, EJ);
-      const double yI = ldexp(yn, c0 - EI);
-      const double zJ = ldexp(zn, c0 - EJ);
-      const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
-      const int ea = pow2_exp(protect_update(tI, zJ, yI));
-      const int ec = c0 + ea;
-      P.SY[(int64_t)I * m + k] = ec - EI;
-      P.SZ[(int64_t)I * m + k] = ec - EJ + p;
-      P.E[(int64_t)I * m + k] = ec;
-    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scaling exponents of the panel pass of tile column J for every tile row
+// I < J: consolidate the scales of y_I and Z_J (update-task.c:68-88), then
+// one protection for the whole fused update (update-task.c:103-248):
+// |update(i)| <= (||H(I,:)||_oo + |mu|) ||Z|| + |R~(i)| |Zlast| and
+// |R~(i)| <= ||(H - mu I)(i,:)||_2, so t_I = 2 (||H(I,:)||_oo + |mu|).
+// Runs after the previous panel pass finished (it reads the row-block
+// norms that pass produced) and before this one.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_params(const ProbDev* __restrict__ probs,
+                                                const K1Item* __restrict__ items) {
+  const K1Item it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int J = it.J;
+  if (J == 0) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k = it.k0 + wave;
+  const int m = P.m;
+  if (k >= m) return;
+  const double mu = P.wr[k];
+  const double zn = P.aux[J & 1][4 * k + 2];
+  const int p = (int)P.aux[J & 1][4 * k + 3];
+  const int EJ = P.E[(int64_t)J * m + k];
+  for (int I = lane; I < J; I += 64) {
+    const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
+    double yn = 0.0;
+    for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, P.Ynb[(int64_t)rb * m + k]);
+    const int EI = P.E[(int64_t)I * m + k];
+    const int c0 = min(EI, EJ);
+    const double yI = ldexp(yn, c0 - EI);
+    const double zJ = ldexp(zn, c0 - EJ);
+    const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
+    const int ea = pow2_exp(protect_update(tI, zJ, yI));
+    const int ec = c0 + ea;
+    P.SY[(int64_t)I * m + k] = ec - EI;
+    P.SZ[(int64_t)I * m + k] = ec - EJ + p;
+    P.E[(int64_t)I * m + k] = ec;
   }
 }
 
@@ -539,15 +579,32 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
 // Block = 64 rows (one row block, inside one tile row I) x 64 columns of
 // [W | Z] interleaved (32 shifts).  Four waves, 32 x 32 each.
 // ---------------------------------------------------------------------------
-constexpr int KC = 32;   // K chunk staged in LDS
-constexpr int LDA = 64 + 2;
-constexpr int LDB = 64 + 2;
+constexpr int KC = 32;          // K chunk staged in LDS
+constexpr int GM = kRowBlock;   // 64 rows per block (one row block)
+constexpr int GN = kGemmCols;   // 64 columns per block (32 shifts)
+constexpr int LDA = GM + 4;
+constexpr int LDB = GN + 4;
+constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 
+// Row maximum over the 16 lanes of each DPP row, in every lane of the row.
+__device__ __forceinline__ double row16_max(double x) {
+  x = fmax(x, dpp_move<0xB1>(x, x));    // quad_perm [1,0,3,2]
+  x = fmax(x, dpp_move<0x4E>(x, x));    // quad_perm [2,3,0,1]
+  x = fmax(x, dpp_move<0x124>(x, x));   // row_ror:4
+  x = fmax(x, dpp_move<0x128>(x, x));   // row_ror:8
+  return x;
+}
+
+// Block = one 64-row block x 64 columns of [W | Z] (32 shifts); four waves,
+// 32 rows x 32 columns each (2 x 2 MFMA tiles).
 __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
                                                 const GemmItem* __restrict__ items) {
-  __shared__ double As[KC * LDA];   // As[k * LDA + row]
-  __shared__ double Bs[KC * LDB];   // Bs[k * LDB + col]
-  __shared__ double red[2][64];
+  __shared__ double As[2][KC * LDA];   // As[buf][k * LDA + row]
+  __shared__ double Bs[2][KC * LDB];   // Bs[buf][k * LDB + col]
+  // epilogue parameters of the block's shifts (loaded during the K loop)
+  __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
+  __shared__ int psy[GN / 2], psz[GN / 2];
+  __shared__ double red[2][GN];
 
   const GemmItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
@@ -556,113 +613,162 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   const int l = tile_first_row(P, J);
   int tile, r0, nr;
   rowblock_range(P, it.rb, tile, r0, nr);
-  const int col0 = it.cb * kGemmCols;
-  const int ncols = min(kGemmCols, 2 * m - col0);
+  const int col0 = it.cb * GN;
+  const int ncols = min(GN, 2 * m - col0);
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
+  const int64_t ldH = P.ldH;
 
-  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * P.ldH;   // panel column 0 = l-1
-  const double* __restrict__ Bg = P.Bm;
+  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;   // panel column 0 = l-1
+  const double* __restrict__ Bg = P.Bm[J & 1];
 
-  double4_t acc[2][2];
+  // global -> register staging of one K chunk
+  constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
+  double pa[NPA], pb[NPB];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int i = 0; i < NPA; ++i) {
+      const int idx = tid + 256 * i;
+      const int kk = idx / GM, rr = idx - kk * GM;
+      pa[i] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) {
+      const int idx = tid + 256 * i;
+      const int cc = idx / KC, k2 = idx - cc * KC;
+      pb[i] = (cc < ncols) ? Bg[(kc + k2) + (int64_t)(col0 + cc) * b] : 0.0;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NPA; ++i) {
+      const int idx = tid + 256 * i;
+      const int kk = idx / GM, rr = idx - kk * GM;
+      As[buf][kk * LDA + rr] = pa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) {
+      const int idx = tid + 256 * i;
+      const int cc = idx / KC, k2 = idx - cc * KC;
+      Bs[buf][k2 * LDB + cc] = pb[i];
+    }
+  };
+
+  // D^T = [W|Z]^T H_panel^T: MFMA operand A = B-matrix values, operand B = H
+  // values; the result D^T[col][row] sits in register col/4 of lane
+  // row + 16 (col % 4): one lane = one matrix row, four columns per tile.
+  double4_t acc[2][CT];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < CT; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
-  for (int kc = 0; kc < b; kc += KC) {
-#pragma unroll
-    for (int i = 0; i < (KC * 64) / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int kk = idx >> 6, rr = idx & 63;
-      As[kk * LDA + rr] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * P.ldH] : 0.0;
+  const int nchunks = b / KC;
+  fetch(0);
+  if (tid < GN / 2) {
+    const int k = col0 / 2 + tid;
+    if (k < m) {
+      const double* aux = P.aux[J & 1];
+      pmu[tid] = P.wr[k];
+      pP[tid] = aux[4 * k + 0];
+      pZl[tid] = aux[4 * k + 1];
+      pb0w[tid] = Bg[(int64_t)(2 * k) * b];
+      pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
+      psy[tid] = P.SY[(int64_t)tile * m + k];
+      psz[tid] = P.SZ[(int64_t)tile * m + k];
     }
-#pragma unroll
-    for (int i = 0; i < (KC * 64) / 256; ++i) {
-      const int idx = tid + 256 * i;
-      const int cc = idx / KC, kk = idx - cc * KC;
-      Bs[kk * LDB + cc] = (cc < ncols) ? Bg[(kc + kk) + (int64_t)(col0 + cc) * b] : 0.0;
-    }
-    __syncthreads();
+  }
+  stash(0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
 #pragma unroll
     for (int k4 = 0; k4 < KC; k4 += 4) {
       const int kr = k4 + (lane >> 4);
-      double av[2], bv[2];
+      double hv[2], zv[CT];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) av[a] = As[kr * LDA + wm * 32 + a * 16 + (lane & 15)];
+      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) bv[c] = Bs[kr * LDB + wn * 32 + c * 16 + (lane & 15)];
+      for (int c = 0; c < CT; ++c)
+        zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+        for (int c = 0; c < CT; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
     }
+    if (ch + 1 < nchunks) stash(buf ^ 1);
     __syncthreads();
   }
 
-  // Epilogue.  v_mfma_f64_16x16x4_f64 result layout: D[i][j] is register
-  // i / 4 of lane j + 16 (i % 4), so acc[a][c][r] holds
-  //   row wm*32 + a*16 + 4*r + lane/16,  col wn*32 + c*16 + lane%16.
+  // Epilogue.  acc[a][c][r]: row  wm*32 + a*16 + lane%16,
+  //                          col  wn*GN/2 + c*16 + 4*r + lane/16.
+  // Even lane groups hold W columns (R~ out), odd ones Z columns (Y); all
+  // row loads are issued before any use.
   const double* __restrict__ Rin = P.Rt[J & 1];
   double* __restrict__ Rout = P.Rt[(J + 1) & 1];
-  double cmax[2] = {0.0, 0.0};
+  const int g = lane >> 4;
+  const bool isz = (g & 1) != 0;
+  const int rl = lane & 15;
+  double rin[CT][4][2], yv[CT][4][2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int cl = wn * 32 + c * 16 + (lane & 15);
-    const int col = col0 + cl;
-    if (cl < ncols) {
-      const int k = col >> 1;
-      const bool isz = col & 1;
-      const double mu = P.wr[k];
-      const double coef = P.aux[2 * k + (isz ? 1 : 0)];
-      const double b0 = Bg[(int64_t)col * b];
-      const int sy = P.SY[(int64_t)tile * m + k];
-      const int sz = P.SZ[(int64_t)tile * m + k];
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
+      const int k = (col0 + cl) >> 1;
+      const bool okc = cl < ncols;
 #pragma unroll
       for (int a = 0; a < 2; ++a) {
+        const int row = wm * 32 + a * 16 + rl;
+        const int gi = r0 + row;
+        const bool ok = okc && row < nr;
+        rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+        yv[c][r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+      }
+    }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = wm * 32 + a * 16 + 4 * r + (lane >> 4);
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
+      const int ks = cl >> 1;
+      const int k = (col0 + cl) >> 1;
+      const bool okc = cl < ncols;
+      double cmax = 0.0;
+      if (okc) {
+        const double mu = pmu[ks];
+        const double coef = isz ? pZl[ks] : pP[ks];
+        const double b0 = isz ? pb0z[ks] : pb0w[ks];
+        const int sy = psy[ks], sz = psz[ks];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const int row = wm * 32 + a * 16 + rl;
+          const int gi = r0 + row;
           if (row < nr) {
-            const int gi = r0 + row;
-            const double rin = Rin[gi + (int64_t)k * n];
-            double val = acc[a][c][r] + coef * rin;
+            double val = acc[a][c][r] + coef * rin[c][r][a];
             if (gi == l - 1) val -= mu * b0;
             if (!isz) {
               Rout[gi + (int64_t)k * n] = val;
             } else {
-              double* yp = P.Y + gi + (int64_t)k * P.ldY;
-              const double ynew = ldexp(*yp, sy) - ldexp(val, sz);
-              *yp = ynew;
-              cmax[c] = fmax(cmax[c], fabs(ynew));
+              const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
+              P.Y[gi + (int64_t)k * P.ldY] = ynew;
+              cmax = fmax(cmax, fabs(ynew));
             }
           }
         }
       }
+      const double x = row16_max(cmax);
+      if (rl == 0) red[wm][cl] = x;
     }
-  }
-  // Column maxima over the block's rows: lanes l, l+16, l+32, l+48 share a
-  // column; then the two row-waves through LDS.
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    double x = cmax[c];
-    x = fmax(x, __shfl_xor(x, 16, 64));
-    x = fmax(x, __shfl_xor(x, 32, 64));
-    cmax[c] = x;
-  }
-  if (lane < 16) {
-#pragma unroll
-    for (int c = 0; c < 2; ++c) red[wm][wn * 32 + c * 16 + lane] = cmax[c];
-  }
   __syncthreads();
-  if (tid < 64) {
+  if (tid < GN) {
     const int cl = tid;
-    const int col = col0 + cl;
-    if (cl < ncols && (col & 1)) {
-      const int k = col >> 1;
+    if (cl < ncols && (cl & 1)) {
+      const int k = (col0 + cl) >> 1;
       P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
     }
   }
@@ -915,6 +1021,12 @@ hipError_t launch_diag(int rpl, const ProbDev* d_probs, const K1Item* items, int
     case 4: hipLaunchKernelGGL(k_diag<4>, dim3(count), dim3(256), 0, st, d_probs, items); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_params, dim3(count), dim3(256), 0, st, d_probs, items);
   return hipGetLastError();
 }
 

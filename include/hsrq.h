@@ -51,6 +51,9 @@ void hsrq_context_destroy(hsrq_context *ctx);
 int hsrq_context_set_tile(hsrq_context *ctx, int tlsz);
 /* Force exact norms in the diagonal sweep (testing aid; default 0). */
 int hsrq_context_set_exact_norms(hsrq_context *ctx, int on);
+/* 1: run the panel pass of the rows above the next diagonal tile on a
+ * second stream, overlapping the next diagonal sweep (default 0). */
+int hsrq_context_set_lookahead(hsrq_context *ctx, int on);
 /* Panel-pass precision, HSRQ_PREC_FP64 (default) or HSRQ_PREC_FP32. */
 int hsrq_context_set_precision(hsrq_context *ctx, int prec);
 const char *hsrq_strerror(int code);
@@ -88,9 +91,10 @@ int hsrq_plan_execute(hsrq_plan *plan);
 /* Launch the plan as a captured HIP graph (captured on first use). */
 int hsrq_plan_execute_graph(hsrq_plan *plan);
 /* Synchronous execution with HIP events around every launch:
- * kernel_ms[5] / launches[5] receive the summed device time and launch count
+ * kernel_ms[6] / launches[6] receive the summed device time and launch count
  * per kernel kind: 0 norms, 1 init, 2 diagonal sweep, 3 panel update,
- * 4 back-transform. */
+ * 4 back-transform, 5 update scaling exponents.  Kernels of the two streams
+ * overlap, so the sum can exceed the wall time. */
 int hsrq_plan_execute_profiled(hsrq_plan *plan, double *kernel_ms, int *launches);
 int hsrq_plan_sync(hsrq_plan *plan);
 /* Copies the per-shift status bits (sum(m) ints) to host. */

@@ -56,6 +56,64 @@ def cpu_baseline(H, shifts, tile, threads):
                       f"shift tile 16, {dt:.2f} s wall"}
 
 
+def batched(ctx, steps, warmup, tile, plain_sample):
+    """Batch workload: all problems in one device schedule vs one at a time."""
+    import torch
+    from hsrq3in_amd import HSRQ_CONVERGED, colmajor_to_device, workloads
+    wl = workloads.BatchWorkload()
+    t0 = time.perf_counter()
+    items = []
+    for p in range(len(wl)):
+        H, s = wl.problem(p)
+        n = H.shape[0]
+        items.append((colmajor_to_device(H), torch.from_numpy(s.copy()).cuda(),
+                      torch.zeros((len(s), n), dtype=torch.float64, device="cuda")))
+    print(f"[bench] batch of {len(items)} problems built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    total_m = sum(w.numel() for _, w, _ in items)
+    plan = ctx.plan(items, tile)
+    for _ in range(warmup):
+        plan.execute(graph=True)
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(graph=True)
+        plan.sync()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    st = plan.status()
+    good, pos, worst = 0, 0, 0.0
+    for Ht, wrt, vrt in items:
+        k = wrt.numel()
+        res = ctx.residuals(Ht, wrt, vrt)
+        n = Ht.shape[0]
+        ok = (res <= 10 * n) & ((st[pos:pos + k] & HSRQ_CONVERGED) != 0)
+        good += int(ok.sum())
+        worst = max(worst, float(np.max(res / (10 * n))))
+        pos += k
+    finite = all(bool(torch.isfinite(v).all().item()) for _, _, v in items)
+    plan.close()
+    # plain path, one problem at a time (device-resident drop-in per problem)
+    sample = items[::max(1, len(items) // plain_sample)]
+    for Ht, wrt, vrt in sample[:2]:
+        ctx.solve(Ht, wrt, vrt, tile)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sm = 0
+    for Ht, wrt, vrt in sample:
+        ctx.solve(Ht, wrt, vrt, tile)
+        sm += wrt.numel()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    batched_rate = good / (ms / 1e3)
+    plain_rate = sm / plain_s
+    return {"value": round(batched_rate, 1), "unit": "eigvecs/s", "problems": len(items),
+            "eigvecs": total_m, "residual_ok": good, "finite": finite,
+            "worst_residual_over_bound": worst, "ms_per_batch": round(ms, 3), "tile": tile,
+            "plain_one_at_a_time": {"value": round(plain_rate, 1), "problems": len(sample),
+                                    "eigvecs": sm, "seconds": round(plain_s, 3)},
+            "batched_over_plain": round(batched_rate / plain_rate, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +124,10 @@ def main():
     ap.add_argument("--m", type=int, default=800)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-tile", type=int, default=64)
+    ap.add_argument("--packing", type=int, default=0)
+    ap.add_argument("--batch-packing", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -85,7 +147,7 @@ def main():
         H0 = workloads.gaussian_hessenberg(n, seed)
         shifts = workloads.real_shifts(H0, m)
     H = workloads.gaussian_hessenberg(n, seed)
-    ctx = Context(0)
+    ctx = Context(0, packing=args.packing)
     ctx.selftest()
     Ht = colmajor_to_device(H)
     wrt = torch.from_numpy(shifts.copy()).cuda()
@@ -142,6 +204,10 @@ def main():
         "kernels": kernels, "roofline": roof,
     }
     plan.close()
+    if not args.no_batch:
+        ctx.set_packing(args.batch_packing)
+        out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 32)
+        out["batched"]["packing"] = args.batch_packing
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(H, shifts, 64, threads)

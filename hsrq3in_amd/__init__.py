@@ -51,6 +51,7 @@ EXPORTS = [
     "tiled_inverse_iteration",
     "hsrq_context_create", "hsrq_context_destroy", "hsrq_context_set_tile",
     "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_context_set_lookahead",
+    "hsrq_context_set_packing",
     "hsrq_strerror",
     "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
@@ -81,6 +82,7 @@ def lib():
         L.hsrq_context_set_exact_norms.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_precision.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_lookahead.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_set_packing.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_stream.restype = _vp
         L.hsrq_context_stream.argtypes = [_vp]
         L.hsrq_selftest.argtypes = [_vp]
@@ -157,13 +159,17 @@ def colmajor_to_host(t) -> np.ndarray:
 class Context:
     """Device, stream and settings of libhsrq (hsrq_context)."""
 
-    def __init__(self, device: int = 0, tile: int = 0, exact_norms: bool = False):
+    def __init__(self, device: int = 0, tile: int = 0, exact_norms: bool = False,
+                 packing: int = 0):
         self._h = _vp()
+        self.packing = packing
         _check(lib().hsrq_context_create(ctypes.byref(self._h), device), "hsrq_context_create")
         if tile:
             _check(lib().hsrq_context_set_tile(self._h, tile), "set_tile")
         if exact_norms:
             _check(lib().hsrq_context_set_exact_norms(self._h, 1), "set_exact_norms")
+        if packing:
+            _check(lib().hsrq_context_set_packing(self._h, packing), "set_packing")
 
     def close(self):
         if self._h:
@@ -182,6 +188,10 @@ class Context:
 
     def set_exact_norms(self, on: bool):
         _check(lib().hsrq_context_set_exact_norms(self._h, int(on)), "set_exact_norms")
+
+    def set_packing(self, shifts_per_wave: int):
+        _check(lib().hsrq_context_set_packing(self._h, shifts_per_wave), "set_packing")
+        self.packing = shifts_per_wave
 
     def set_lookahead(self, on: bool):
         _check(lib().hsrq_context_set_lookahead(self._h, int(on)), "set_lookahead")

@@ -23,6 +23,7 @@ hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
 hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
+hipError_t launch_diag16(int, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);
 hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
@@ -53,6 +54,7 @@ struct hsrq_context {
   int exact = 0;
   int precision = HSRQ_PREC_FP64;
   int lookahead = 0;     // 1: bulk panel pass on a second stream
+  int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
 };
 
 namespace {
@@ -93,8 +95,9 @@ struct hsrq_plan {
   int n_init_items = 0;
   const ShiftItem* d_bt_items = nullptr;
   int n_bt_items = 0;
-  std::vector<const K1Item*> k1_ptr;
-  std::vector<int> k1_cnt;
+  std::vector<const K1Item*> k1_ptr, pr_ptr;
+  std::vector<int> k1_cnt, pr_cnt;
+  int pack = 1;   // shifts per wavefront of the diagonal sweep
   // panel passes: rows of the next diagonal tile ("a", on the main stream,
   // it gates the next diagonal sweep) and all rows above it ("b", on the
   // side stream, overlapping the next diagonal sweep)
@@ -247,7 +250,13 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     for (int k = 0; k < P.m; ++k) bt_items.push_back({i, k, 0, 0});
   }
   p->steps = steps;
-  std::vector<std::vector<K1Item>> k1(steps);
+  std::vector<std::vector<K1Item>> k1(steps), pr(steps);
+  // packed sweep (4 shifts per wavefront, 16 per workgroup) unless asked
+  // otherwise: it issues ~4x fewer instructions per shift step; one shift
+  // per wavefront keeps the shortest latency per step for single problems.
+  p->pack = ctx->pack ? ctx->pack : 4;
+  if (b % 64 != 0 || (b / 16 & (b / 16 - 1)) != 0) p->pack = 1;   // 16 lanes x 2^k rows
+  const int k1step = p->pack == 4 ? 16 : kShiftsPerK1;
   std::vector<std::vector<GemmItem>> ga(steps), gb(steps);
   for (int s = 0; s < steps; ++s) {
     for (int i = 0; i < count; ++i) {
@@ -255,7 +264,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       if (P.m == 0) continue;
       const int J = P.T - 1 - s;
       if (J < 0) continue;
-      for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) k1[s].push_back({i, J, k0, 0});
+      for (int k0 = 0; k0 < P.m; k0 += k1step) k1[s].push_back({i, J, k0, 0});
+      for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) pr[s].push_back({i, J, k0, 0});
       if (J >= 1) {
         // rows of the next diagonal tile (part a) and all rows above (part
         // b), in pairs of row blocks; XCD-aware order: blocks are dealt to
@@ -287,6 +297,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   nbytes += align_up(bt_items.size() * sizeof(ShiftItem));
   for (int s = 0; s < steps; ++s) {
     nbytes += align_up(k1[s].size() * sizeof(K1Item));
+    nbytes += align_up(pr[s].size() * sizeof(K1Item));
     nbytes += align_up(ga[s].size() * sizeof(GemmItem));
     nbytes += align_up(gb[s].size() * sizeof(GemmItem));
   }
@@ -315,6 +326,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   p->n_bt_items = (int)bt_items.size();
   p->k1_ptr.resize(steps);
   p->k1_cnt.resize(steps);
+  p->pr_ptr.resize(steps);
+  p->pr_cnt.resize(steps);
   p->ga_ptr.resize(steps);
   p->ga_cnt.resize(steps);
   p->gb_ptr.resize(steps);
@@ -322,6 +335,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   for (int s = 0; s < steps; ++s) {
     p->k1_ptr[s] = (const K1Item*)put(k1[s].data(), k1[s].size() * sizeof(K1Item));
     p->k1_cnt[s] = (int)k1[s].size();
+    p->pr_ptr[s] = (const K1Item*)put(pr[s].data(), pr[s].size() * sizeof(K1Item));
+    p->pr_cnt[s] = (int)pr[s].size();
     p->ga_ptr[s] = (const GemmItem*)put(ga[s].data(), ga[s].size() * sizeof(GemmItem));
     p->ga_cnt[s] = (int)ga[s].size();
     p->gb_ptr[s] = (const GemmItem*)put(gb[s].data(), gb[s].size() * sizeof(GemmItem));
@@ -400,13 +415,14 @@ int plan_enqueue_impl(hsrq_plan* p, Prof* prof) {
   int last_b = -1;
   for (int s = 0; s < p->steps; ++s) {
     rc = run(2, st, [&](hipStream_t x) {
-      return launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
+      return p->pack == 4 ? launch_diag16(p->b / 16, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x)
+                          : launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
     });
     if (rc) return rc;
     if (p->ga_cnt[s] + p->gb_cnt[s] == 0) continue;
     if (last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, p->ev_b[last_b], 0));
     rc = run(5, st, [&](hipStream_t x) {
-      return launch_params(p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
+      return launch_params(p->d_probs, p->pr_ptr[s], p->pr_cnt[s], x);
     });
     if (rc) return rc;
     HIP_TRY(hipEventRecord(p->ev_p[s], st));
@@ -579,6 +595,13 @@ int hsrq_context_set_tile(hsrq_context* ctx, int tlsz) {
 int hsrq_context_set_exact_norms(hsrq_context* ctx, int on) {
   if (!ctx) return HSRQ_EINVAL;
   ctx->exact = on ? 1 : 0;
+  return HSRQ_OK;
+}
+
+int hsrq_context_set_packing(hsrq_context* ctx, int shifts_per_wave) {
+  if (!ctx || (shifts_per_wave != 0 && shifts_per_wave != 1 && shifts_per_wave != 4))
+    return HSRQ_EINVAL;
+  ctx->pack = shifts_per_wave;
   return HSRQ_OK;
 }
 

@@ -162,6 +162,42 @@ __device__ __forceinline__ double wave_max(double x) {
   return readlane(x, 63);
 }
 
+// Maximum over the 16 lanes of each DPP row, in every lane of the row.
+__device__ __forceinline__ double row16_max(double x) {
+  x = fmax(x, dpp_move<0xB1>(x, x));    // quad_perm [1,0,3,2]
+  x = fmax(x, dpp_move<0x4E>(x, x));    // quad_perm [2,3,0,1]
+  x = fmax(x, dpp_move<0x124>(x, x));   // row_ror:4
+  x = fmax(x, dpp_move<0x128>(x, x));   // row_ror:8
+  return x;
+}
+
+// Lane L of every DPP row (16 lanes) broadcast to the whole row.
+template <int L>
+__device__ __forceinline__ double bcast16(double x) {
+  return dpp_move<0x150 + L>(x, x);   // row_newbcast:L
+}
+
+__device__ __forceinline__ double bcast16_rt(double x, int l) {
+  switch (l & 15) {
+    case 0: return bcast16<0>(x);
+    case 1: return bcast16<1>(x);
+    case 2: return bcast16<2>(x);
+    case 3: return bcast16<3>(x);
+    case 4: return bcast16<4>(x);
+    case 5: return bcast16<5>(x);
+    case 6: return bcast16<6>(x);
+    case 7: return bcast16<7>(x);
+    case 8: return bcast16<8>(x);
+    case 9: return bcast16<9>(x);
+    case 10: return bcast16<10>(x);
+    case 11: return bcast16<11>(x);
+    case 12: return bcast16<12>(x);
+    case 13: return bcast16<13>(x);
+    case 14: return bcast16<14>(x);
+    default: return bcast16<15>(x);
+  }
+}
+
 __device__ __forceinline__ double wave_sum(double x) {
   x = x + dpp_move<0xB1>(0.0, x);
   x = x + dpp_move<0x4E>(0.0, x);

@@ -18,6 +18,8 @@
 #include <float.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "hsrq_device.h"
 #include "hsrq_internal.h"
 
@@ -152,13 +154,22 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
 // ---------------------------------------------------------------------------
 // K1: diagonal tile J, one wavefront per shift, RPL rows per lane.
 // ---------------------------------------------------------------------------
+// a[q] for a uniform slot q, as a binary tree of selects on the bits of q
+// (keeps the register array out of memory).
+template <int N>
+__device__ __forceinline__ double pick_tree(const double* a, int q, int bit) {
+  if constexpr (N == 1) {
+    return a[0];
+  } else {
+    const double lo = pick_tree<N / 2>(a, q, bit >> 1);
+    const double hi = pick_tree<N / 2>(a + N / 2, q, bit >> 1);
+    return (q & bit) ? hi : lo;
+  }
+}
 template <int RPL>
 __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
-  double t = a[0];
-#pragma unroll
-  for (int i = 1; i < RPL; ++i)
-    if (q == i) t = a[i];
-  return t;
+  static_assert((RPL & (RPL - 1)) == 0, "power of two");
+  return pick_tree<RPL>(a, q, RPL / 2);
 }
 
 template <int RPL>
@@ -280,15 +291,15 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     o.s = s;
     o.r = r;
     o.rinv = rinv;
-    o.br = (fabs(c) * Bv + fabs(s) * Hm) * slack;
-    o.bv = (fabs(c) * Hm + fabs(s) * Bv) * slack;
+    o.br = fma(fabs(c), Bv, fabs(s) * Hm) * slack;
+    o.bv = fma(fabs(c), Hm, fabs(s) * Bv) * slack;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = lane + 64 * q;
       const bool act = row <= st - 1;
       const double h = (row == st - 1) ? hdd : hc[row];
-      nr[q] = act ? c * v[q] - s * h : 0.0;
-      nv[q] = act ? c * h + s * v[q] : v[q];
+      nr[q] = act ? fma(c, v[q], -(s * h)) : 0.0;
+      nv[q] = act ? fma(c, h, s * v[q]) : v[q];
     }
     return rare;
   };
@@ -303,12 +314,12 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     const double acsr = fabs(o.r);
     bool rare = !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
     const double yq = yj * o.rinv;
-    byn = (By + fabs(yq) * o.br) * slack;
+    byn = fma(fabs(yq), o.br, By) * slack;
     rare = rare || !(byn <= kOmega);
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = lane + 64 * q;
-      ny[q] = (row <= st - 1) ? y[q] - yq * r[q] : ((row == st) ? yq : y[q]);
+      ny[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
     }
     return rare;
   };
@@ -341,7 +352,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     double a = 0.0;
 #pragma unroll
     for (int q = 0; q < RPL; ++q)
-      if (lane + 64 * q <= st - 1) a = fmax(a, ldexp(fabs(y[q]), -kk) + ays * fabs(r[q]));
+      if (lane + 64 * q <= st - 1) a = fmax(a, fma(ays, fabs(r[q]), ldexp(fabs(y[q]), -kk)));
     a = wave_max(a);
     int e2 = 0;
     if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
@@ -354,20 +365,18 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = lane + 64 * q;
-      if (row <= st - 1) y[q] = y[q] - yj * r[q];
+      if (row <= st - 1) y[q] = fma(-yj, r[q], y[q]);
       if (row == st) y[q] = yj;
     }
     By = ldexp(a, kk + e2) * slack;
   };
 
   auto keep_rotation = [&](int st, const VOut& o) {
-    if (lane == (st & 63)) {
+    const int own = (lane == (st & 63)) ? (st >> 6) : -1;
 #pragma unroll
-      for (int q = 0; q < RPL; ++q)
-        if (q == (st >> 6)) {
-          cr[q] = o.c;
-          sr[q] = o.s;
-        }
+    for (int q = 0; q < RPL; ++q) {
+      cr[q] = (own == q) ? o.c : cr[q];
+      sr[q] = (own == q) ? o.s : sr[q];
     }
   };
 
@@ -478,29 +487,55 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   double w[RPL], z[RPL];
   double carry_p = 1.0, carry_t = 0.0;
   double zmax = 0.0;
+  // Scans run over segments of 16 rows with carries taken segment by segment
+  // in row order -- the association of the packed kernel, so that both
+  // layouts produce bit-identical coefficients.
+  const int sg16 = lane >> 4, t16 = lane & 15;
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int r = lane + 64 * q;
     const bool in = r < nb;
     const double sq = in ? sr[q] : 1.0;
     const double cq = in ? cr[q] : 0.0;
-    // exclusive prefix product of s
-    const double inc = wave_prefix_prod(sq);
-    double ex = __shfl_up(inc, 1, 64);
-    if (lane == 0) ex = 1.0;
-    w[q] = cq * (ex * carry_p);
-    carry_p *= readlane(inc, 63);
-    // affine scan for t
+    double inc = sq;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const double pv = __shfl_up(inc, off, 16);
+      if (t16 >= off) inc = inc * pv;
+    }
+    double ex = __shfl_up(inc, 1, 16);
+    if (t16 == 0) ex = 1.0;
     double a = in ? sq : 1.0, bb = in ? cq * y[q] : 0.0;
-    wave_affine_scan(a, bb);
-    const double t = a * carry_t + bb;   // t_r
-    double tprev = __shfl_up(t, 1, 64);
-    if (lane == 0) tprev = carry_t;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const double pa = __shfl_up(a, off, 16);
+      const double pb = __shfl_up(bb, off, 16);
+      if (t16 >= off) {
+        bb = a * pb + bb;
+        a = a * pa;
+      }
+    }
+    double my_cp = carry_p, my_ct = carry_t;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (sg16 == g) {
+        my_cp = carry_p;
+        my_ct = carry_t;
+      }
+      const double segp = readlane(inc, 16 * g + 15);
+      const double sega = readlane(a, 16 * g + 15);
+      const double segb = readlane(bb, 16 * g + 15);
+      carry_p *= segp;
+      carry_t = sega * carry_t + segb;
+    }
+    w[q] = cq * (ex * my_cp);
+    const double tt = a * my_ct + bb;
+    double tprev = __shfl_up(tt, 1, 16);
+    if (t16 == 0) tprev = my_ct;
     if (r == 0)
       z[q] = -sq * y[q];
     else
       z[q] = in ? cq * tprev - sq * y[q] : 0.0;
-    carry_t = readlane(t, 63);
     zmax = fmax(zmax, fabs(z[q]));
   }
   const double Pk = carry_p;
@@ -526,6 +561,444 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       }
     }
     if (lane == 0) {
+      P.aux[J & 1][4 * k + 0] = Pk;
+      P.aux[J & 1][4 * k + 1] = ldexp(zlast, -p);
+      P.aux[J & 1][4 * k + 2] = zn;
+      P.aux[J & 1][4 * k + 3] = (double)p;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1, packed form: 16 lanes per shift, 4 shifts per wavefront, 16 per
+// workgroup.  Same algorithm and arithmetic as k_diag; the per-shift scalar
+// chain (rotation, pivot, protections) is executed once for four shifts,
+// broadcasts inside a shift's 16-lane DPP row use row_newbcast, so the hot
+// loop is unrolled by 16 to give every broadcast a compile-time source lane.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ double bc16(double x, int lrt) {
+  if constexpr (L < 0)
+    return bcast16_rt(x, lrt);
+  else
+    return bcast16<L>(x);
+}
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <int RPL>
+__global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ probs,
+                                                const K1Item* __restrict__ items) {
+  constexpr int BMAX = 16 * RPL;
+  constexpr int CW = (BMAX <= 64) ? 64 : (BMAX <= 128 ? 32 : 16);   // LDS chunk columns
+  __shared__ double Hs[CW * BMAX];
+
+  const K1Item it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int J = it.J;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int grp = lane >> 4, t = lane & 15;
+  const int m = P.m, n = P.n;
+  const int kraw = it.k0 + wave * 4 + grp;
+  const bool valid = kraw < m;
+  const int k = valid ? kraw : m - 1;
+  const int l = tile_first_row(P, J), nb = tile_rows(P, J);
+  const double mu = P.wr[k];
+  const double* __restrict__ H = P.H;
+  const int64_t ldH = P.ldH;
+  const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
+  double* yg = P.Y + (int64_t)k * P.ldY + l;
+
+  double v[RPL], y[RPL], cr[RPL], sr[RPL];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = t + 16 * q;
+    v[q] = r < nb ? Rin[r] : 0.0;
+    y[q] = r < nb ? yg[r] : 0.0;
+    cr[q] = 1.0;
+    sr[q] = 0.0;
+  }
+  int E = P.E[(int64_t)J * m + k];
+  const double Hnorm = P.diagn[J];
+  const double eps = 0x1p-53;
+  const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
+  const double smin = fmax(eps * fabs(mu), smlnum);                        // solve-task.c:47
+  int info = 0;
+  const double slack = 1.0 + 0x1p-40;
+  const bool exact = P.force_exact != 0;
+
+  constexpr int NL = CW * BMAX / 256;
+  double pf[NL];
+  auto issue = [&](int chunk) __attribute__((always_inline)) {
+    const int c0 = chunk * CW;
+    const int nc = min(CW, nb - c0);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int c = idx / BMAX, r = idx - c * BMAX;
+      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+    }
+  };
+  int cur = -1;
+  auto column = [&](int col) __attribute__((always_inline)) -> const double* {
+    const int ch = col / CW;
+    if (ch != cur) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < NL; ++i) Hs[threadIdx.x + 256 * i] = pf[i];
+      __syncthreads();
+      if (ch > 0) issue(ch - 1);
+      cur = ch;
+    }
+    return &Hs[(col - ch * CW) * BMAX];
+  };
+
+  double By, Bv;
+  {
+    double ly = 0.0, lv = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      ly = fmax(ly, fabs(y[q]));
+      lv = fmax(lv, fabs(v[q]));
+    }
+    By = row16_max(ly);
+    Bv = row16_max(lv);
+  }
+
+  struct VOut {
+    double c, s, r, rinv, br, bv;
+  };
+  // v-chain step st (see k_diag); Lc: compile-time lane of row st, or -1.
+  auto vstep = [&](auto Lc, int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
+                   double (&nv)[RPL]) __attribute__((always_inline)) -> bool {
+    constexpr int L = decltype(Lc)::value;
+    const double vs = bc16<L>(pick<RPL>(v, st >> 4), st & 15);
+    const double hsub = hc[st];
+    const double hdd = hc[st - 1] - mu;
+    double c, s, r, rinv;
+    bool rare = false;
+    if (full) {
+      givens(vs, hsub, c, s);
+      r = c * vs - s * hsub;
+      rinv = 1.0 / r;
+    } else {
+      const double f1 = fabs(vs), g1 = fabs(hsub);
+      rare = !(f1 > 0x1p-511 && f1 < 0x1.6a09e667f3bcdp+510 && g1 > 0x1p-511 &&
+               g1 < 0x1.6a09e667f3bcdp+510);
+      const double d2 = fma(vs, vs, hsub * hsub);
+      const double id = rsqrt_nr(d2);
+      rinv = copysign(id, vs);
+      r = d2 * rinv;
+      c = vs * rinv;
+      s = -(hsub * rinv);
+    }
+    const double Hm = fmax(Hnorm, fabs(hdd));
+    o.c = c;
+    o.s = s;
+    o.r = r;
+    o.rinv = rinv;
+    o.br = fma(fabs(c), Bv, fabs(s) * Hm) * slack;
+    o.bv = fma(fabs(c), Hm, fabs(s) * Bv) * slack;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = t + 16 * q;
+      const bool act = row <= st - 1;
+      const double h = (row == st - 1) ? hdd : hc[row];
+      nr[q] = act ? fma(c, v[q], -(s * h)) : 0.0;
+      nv[q] = act ? fma(c, h, s * v[q]) : v[q];
+    }
+    return rare;
+  };
+  auto ystep_fast = [&](auto Lc, int st, const VOut& o, const double (&r)[RPL],
+                        double (&ny)[RPL], double& byn) __attribute__((always_inline)) -> bool {
+    constexpr int L = decltype(Lc)::value;
+    const double yj = bc16<L>(pick<RPL>(y, st >> 4), st & 15);
+    const double acsr = fabs(o.r);
+    bool rare = !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
+    const double yq = yj * o.rinv;
+    byn = fma(fabs(yq), o.br, By) * slack;
+    rare = rare || !(byn <= kOmega);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = t + 16 * q;
+      ny[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
+    }
+    return rare;
+  };
+  auto ystep_slow = [&](int st, const VOut& o, const double (&r)[RPL]) __attribute__((always_inline)) {
+    double yj = bcast16_rt(pick<RPL>(y, st >> 4), st & 15);
+    double csr = o.r, cinv = o.rinv;
+    if (fabs(csr) < smin) {
+      csr = smin;
+      cinv = 1.0 / smin;
+      info = 1;
+    }
+    const int e1 = pow2_exp(protect_real_division(yj, csr));
+    yj = ldexp(yj, e1) * cinv;
+    if (e1 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (t + 16 * q != st) y[q] = ldexp(y[q], e1);
+      E += e1;
+    }
+    const double ay = fabs(yj);
+    const int kk = (ay > 0.0) ? ilogb(ay) : 0;
+    const double ays = ldexp(ay, -kk);
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q)
+      if (t + 16 * q <= st - 1) a = fmax(a, fma(ays, fabs(r[q]), ldexp(fabs(y[q]), -kk)));
+    a = row16_max(a);
+    int e2 = 0;
+    if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
+    if (e2 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e2);
+      yj = ldexp(yj, e2);
+      E += e2;
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = t + 16 * q;
+      if (row <= st - 1) y[q] = fma(-yj, r[q], y[q]);
+      if (row == st) y[q] = yj;
+    }
+    By = ldexp(a, kk + e2) * slack;
+  };
+  auto keep_rotation = [&](int st, const VOut& o) __attribute__((always_inline)) {
+    const int own = (t == (st & 15)) ? (st >> 4) : -1;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      cr[q] = (own == q) ? o.c : cr[q];
+      sr[q] = (own == q) ? o.s : sr[q];
+    }
+  };
+
+  if (nb >= 2) {
+    issue((nb - 2) / CW);
+    VOut cv;
+    double rr[RPL], nr[RPL], nv[RPL], ny[RPL];
+    {
+      const double* hc = column(nb - 2);
+      if (vstep(IC<-1>{}, nb - 1, hc, false, cv, nr, nv))
+        vstep(IC<-1>{}, nb - 1, hc, true, cv, nr, nv);
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        v[q] = nv[q];
+        rr[q] = nr[q];
+      }
+      Bv = cv.bv;
+      keep_rotation(nb - 1, cv);
+    }
+    // Fast step s (y-step s with v-step s-1 interleaved), compile-time lane
+    // LY of row s: commits and returns true unless a shift of this wave needs
+    // a rare case, in which case nothing changed and the general step runs.
+    auto fast_step = [&](auto LYc, int s) __attribute__((always_inline)) -> bool {
+      constexpr int LY = decltype(LYc)::value;
+      constexpr int LV = (LY + 15) & 15;
+      const double* hc = column(s - 2);
+      VOut nvo;
+      double byn;
+      const bool rv = vstep(IC<LV>{}, s - 1, hc, false, nvo, nr, nv);
+      const bool ry = ystep_fast(IC<LY>{}, s, cv, rr, ny, byn);
+      if (__any(rv || ry || exact)) return false;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        y[q] = ny[q];
+        v[q] = nv[q];
+        rr[q] = nr[q];
+      }
+      By = byn;
+      Bv = nvo.bv;
+      cv = nvo;
+      keep_rotation(s - 1, cv);
+      return true;
+    };
+    // General step s with runtime lanes: fast or protected per shift.
+    auto full_step = [&](int s) __attribute__((always_inline)) {
+      const double* hc = column(s - 2);
+      VOut nvo;
+      double byn;
+      if (vstep(IC<-1>{}, s - 1, hc, false, nvo, nr, nv))
+        vstep(IC<-1>{}, s - 1, hc, true, nvo, nr, nv);
+      if (ystep_fast(IC<-1>{}, s, cv, rr, ny, byn) || exact) {
+        ystep_slow(s, cv, rr);
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        By = byn;
+      }
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        v[q] = nv[q];
+        rr[q] = nr[q];
+      }
+      Bv = nvo.bv;
+      cv = nvo;
+      keep_rotation(s - 1, cv);
+    };
+    int s = nb - 1;
+    while (s >= 2) {
+      if ((s & 15) == 15 && s >= 17) {
+        if (!fast_step(IC<15>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<14>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<13>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<12>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<11>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<10>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<9>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<8>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<7>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<6>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<5>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<4>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<3>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<2>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<1>{}, s)) goto general;
+        --s;
+        if (!fast_step(IC<0>{}, s)) goto general;
+        --s;
+        continue;
+      }
+    general:
+      full_step(s);
+      --s;
+    }
+    {
+      double byn;
+      if (ystep_fast(IC<1>{}, 1, cv, rr, ny, byn) || exact) {
+        ystep_slow(1, cv, rr);
+      } else {
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        By = byn;
+      }
+    }
+  }
+
+  // Final rotation and y(0) (solve-task.c:104-118, :212-218).
+  {
+    double v0 = bcast16<0>(v[0]);
+    if (J > 0) {
+      const double hleft = H[l + (int64_t)(l - 1) * ldH];
+      double c0 = 1.0, s0 = 0.0;
+      if (hleft != 0.0) givens(v0, hleft, c0, s0);
+      v0 = v0 * c0 - hleft * s0;
+      if (t == 0) {
+        cr[0] = c0;
+        sr[0] = s0;
+      }
+    }
+    double y0 = bcast16<0>(y[0]);
+    double csr = v0;
+    if (fabs(csr) < smin) {
+      csr = smin;
+      info = 1;
+    }
+    const int e = pow2_exp(protect_real_division(y0, csr));
+    y0 = ldexp(y0, e) / csr;
+    if (e != 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) y[q] = ldexp(y[q], e);
+      E += e;
+    }
+    if (t == 0) y[0] = y0;
+  }
+
+  if (valid) {
+    double* Cg = P.C + (int64_t)k * n + l;
+    double* Sg = P.S + (int64_t)k * n + l;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = t + 16 * q;
+      if (r < nb) {
+        yg[r] = y[q];
+        Cg[r] = cr[q];
+        Sg[r] = sr[q];
+      }
+    }
+    if (t == 0) {
+      P.E[(int64_t)J * m + k] = E;
+      if (info) atomicOr(&P.status[k], 2);
+    }
+  }
+  if (J == 0) return;
+
+  // Coefficients of the panel pass (see k_diag), scans over the 16 lanes of
+  // each shift's DPP row, carried across the RPL row slots.
+  double w[RPL], z[RPL];
+  double carry_p = 1.0, carry_t = 0.0;
+  double zmax = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = t + 16 * q;
+    const bool in = r < nb;
+    const double sq = in ? sr[q] : 1.0;
+    const double cq = in ? cr[q] : 0.0;
+    double inc = sq;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const double pv = __shfl_up(inc, off, 16);
+      if (t >= off) inc = inc * pv;
+    }
+    double ex = __shfl_up(inc, 1, 16);
+    if (t == 0) ex = 1.0;
+    w[q] = cq * (ex * carry_p);
+    carry_p *= bcast16<15>(inc);
+    double a = in ? sq : 1.0, bb = in ? cq * y[q] : 0.0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const double pa = __shfl_up(a, off, 16);
+      const double pb = __shfl_up(bb, off, 16);
+      if (t >= off) {
+        bb = a * pb + bb;
+        a = a * pa;
+      }
+    }
+    const double tt = a * carry_t + bb;
+    double tprev = __shfl_up(tt, 1, 16);
+    if (t == 0) tprev = carry_t;
+    if (r == 0)
+      z[q] = -sq * y[q];
+    else
+      z[q] = in ? cq * tprev - sq * y[q] : 0.0;
+    carry_t = bcast16<15>(tt);
+    zmax = fmax(zmax, fabs(z[q]));
+  }
+  const double Pk = carry_p;
+  const double zlast = carry_t;
+  const double zn = fmax(row16_max(zmax), fabs(zlast));
+  const double hinf = P.scal[0];
+  const double tmax = 2.0 * (hinf + fabs(mu));
+  int p = 0;
+  if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
+
+  if (valid) {
+    double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
+    double* Bz = Bw + P.b;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = t + 16 * q;
+      if (r < nb) {
+        Bw[r] = w[q];
+        Bz[r] = ldexp(z[q], -p);
+      }
+    }
+    if (t == 0) {
       P.aux[J & 1][4 * k + 0] = Pk;
       P.aux[J & 1][4 * k + 1] = ldexp(zlast, -p);
       P.aux[J & 1][4 * k + 2] = zn;
@@ -585,15 +1058,6 @@ constexpr int GN = kGemmCols;   // 64 columns per block (32 shifts)
 constexpr int LDA = GM + 4;
 constexpr int LDB = GN + 4;
 constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
-
-// Row maximum over the 16 lanes of each DPP row, in every lane of the row.
-__device__ __forceinline__ double row16_max(double x) {
-  x = fmax(x, dpp_move<0xB1>(x, x));    // quad_perm [1,0,3,2]
-  x = fmax(x, dpp_move<0x4E>(x, x));    // quad_perm [2,3,0,1]
-  x = fmax(x, dpp_move<0x124>(x, x));   // row_ror:4
-  x = fmax(x, dpp_move<0x128>(x, x));   // row_ror:8
-  return x;
-}
 
 // Block = one 64-row block x 64 columns of [W | Z] (32 shifts); four waves,
 // 32 rows x 32 columns each (2 x 2 MFMA tiles).
@@ -749,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
           const int row = wm * 32 + a * 16 + rl;
           const int gi = r0 + row;
           if (row < nr) {
-            double val = acc[a][c][r] + coef * rin[c][r][a];
+            double val = fma(coef, rin[c][r][a], acc[a][c][r]);
             if (gi == l - 1) val -= mu * b0;
             if (!isz) {
               Rout[gi + (int64_t)k * n] = val;
@@ -893,20 +1357,45 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
 // column i-1 (the reference's dense product of validation.c:48-77 adds
 // zeros below the subdiagonal).
 // ---------------------------------------------------------------------------
+// r(:, k) for 16 shifts and 64 rows per workgroup: x chunks of the 16 shifts
+// are staged in LDS, every H element of the row block is read once per 16
+// shifts (columns from the first row's subdiagonal on).
 __global__ __launch_bounds__(256) void k_resid_rows(int n, const double* __restrict__ H,
-                                                    int64_t ldH, const double* __restrict__ wr,
+                                                    int64_t ldH, int m,
+                                                    const double* __restrict__ wr,
                                                     const double* __restrict__ X, int64_t ldX,
                                                     double* __restrict__ R) {
-  const int k = blockIdx.y;
-  const int row0 = blockIdx.x * 256;
-  const int i = row0 + threadIdx.x;
-  const double* x = X + (int64_t)k * ldX;
-  double acc = 0.0;
+  constexpr int RS = 16, XC = 64;
+  __shared__ double xs[RS][XC];
+  const int row0 = blockIdx.x * 64;
+  const int k0 = blockIdx.y * RS;
+  const int rl = threadIdx.x & 63, sg = threadIdx.x >> 6;   // 4 shift groups of 4
+  const int i = row0 + rl;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int jstart = row0 > 0 ? row0 - 1 : 0;
+  for (int j0 = jstart; j0 < n; j0 += XC) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < RS * XC; t += 256) {
+      const int s = t / XC, jj = t - s * XC;
+      const int k = k0 + s, j = j0 + jj;
+      xs[s][jj] = (k < m && j < n) ? X[j + (int64_t)k * ldX] : 0.0;
+    }
+    __syncthreads();
+    if (i < n) {
+      const int je = min(XC, n - j0);
+      for (int jj = 0; jj < je; ++jj) {
+        const double h = H[i + (int64_t)(j0 + jj) * ldH];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += h * xs[sg * 4 + q][jj];
+      }
+    }
+  }
   if (i < n) {
-    for (int j = jstart; j < n; ++j) acc += H[i + (int64_t)j * ldH] * x[j];
-    acc -= wr[k] * x[i];
-    R[i + (int64_t)k * n] = acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + sg * 4 + q;
+      if (k < m) R[i + (int64_t)k * n] = acc[q] - wr[k] * X[i + (int64_t)k * ldX];
+    }
   }
 }
 
@@ -1024,6 +1513,18 @@ hipError_t launch_diag(int rpl, const ProbDev* d_probs, const K1Item* items, int
   return hipGetLastError();
 }
 
+hipError_t launch_diag16(int rpl, const ProbDev* d_probs, const K1Item* items, int count,
+                         hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  switch (rpl) {
+    case 4: hipLaunchKernelGGL(k_diag16<4>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    case 8: hipLaunchKernelGGL(k_diag16<8>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    case 16: hipLaunchKernelGGL(k_diag16<16>, dim3(count), dim3(256), 0, st, d_probs, items); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count, hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_params, dim3(count), dim3(256), 0, st, d_probs, items);
@@ -1052,7 +1553,8 @@ hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const dou
   double* rs = work + (int64_t)n * m;
   const int nb = (n + 255) / 256;
   hipLaunchKernelGGL(k_rowsums, dim3(nb), dim3(256), 0, st, n, H, ldH, rs);
-  hipLaunchKernelGGL(k_resid_rows, dim3(nb, m), dim3(256), 0, st, n, H, ldH, wr, X, ldX, R);
+  hipLaunchKernelGGL(k_resid_rows, dim3((n + 63) / 64, (m + 15) / 16), dim3(256), 0, st, n, H, ldH,
+                     m, wr, X, ldX, R);
   hipLaunchKernelGGL(k_resid_norms, dim3(m), dim3(256), 0, st, n, (const double*)rs, X, ldX,
                      (const double*)R, res);
   return hipGetLastError();

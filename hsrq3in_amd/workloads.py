@@ -141,3 +141,32 @@ def load_shifts(name: str) -> np.ndarray:
     path = os.path.join(GOLDEN_DIR, name)
     with np.load(path, allow_pickle=False) as z:
         return np.array(z["shifts"])
+
+
+class BatchWorkload:
+    """The 256-problem batch: n log-uniform in [200, 4000], m = 5-20 % of n
+    real-eigenvalue shifts (stored fixture), 10 % graded with k in
+    [-400, 400].  Problem p is rebuilt from its seed on demand."""
+
+    KMAX = 400
+
+    def __init__(self, name: str = "shifts_batch256.npz"):
+        path = os.path.join(GOLDEN_DIR, name)
+        with np.load(path, allow_pickle=False) as z:
+            self.shifts = np.array(z["shifts"])
+            self.offsets = np.array(z["offsets"])
+            self.n = np.array(z["n"])
+            self.m = np.array(z["m"])
+            self.graded = np.array(z["graded"])
+            self.seeds = np.array(z["seeds"])
+
+    def __len__(self):
+        return len(self.n)
+
+    def problem(self, p: int):
+        n, seed = int(self.n[p]), int(self.seeds[p])
+        h = gaussian_hessenberg(n, seed)
+        if self.graded[p]:
+            h = graded(h, self.KMAX, seed)
+        s = self.shifts[self.offsets[p]:self.offsets[p + 1]]
+        return h, np.ascontiguousarray(s)

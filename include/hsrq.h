@@ -51,6 +51,10 @@ void hsrq_context_destroy(hsrq_context *ctx);
 int hsrq_context_set_tile(hsrq_context *ctx, int tlsz);
 /* Force exact norms in the diagonal sweep (testing aid; default 0). */
 int hsrq_context_set_exact_norms(hsrq_context *ctx, int on);
+/* Shifts per wavefront in the diagonal sweep: 1 (64 lanes per shift,
+ * shortest step latency), 4 (16 lanes per shift, ~4x fewer instructions per
+ * shift step: throughput) or 0 (automatic, default). */
+int hsrq_context_set_packing(hsrq_context *ctx, int shifts_per_wave);
 /* 1: run the panel pass of the rows above the next diagonal tile on a
  * second stream, overlapping the next diagonal sweep (default 0). */
 int hsrq_context_set_lookahead(hsrq_context *ctx, int on);

@@ -19,10 +19,11 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-@pytest.fixture(scope="module")
-def ctx():
+@pytest.fixture(scope="module", params=[1, 4], ids=["pack1", "pack4"])
+def ctx(request):
+    """Contexts for both diagonal-sweep layouts: 1 and 4 shifts per wavefront."""
     from hsrq3in_amd import Context
-    c = Context(0)
+    c = Context(0, packing=request.param)
     yield c
     c.close()
 
@@ -91,10 +92,23 @@ def test_exact_norms_identical(ctx):
     from hsrq3in_amd import Context
     H, wr = case(300, seed=11, m=12, kmax=300)
     x1, _, _, _ = gpu_solve(ctx, H, wr, 64)
-    c2 = Context(0, exact_norms=True)
+    c2 = Context(0, exact_norms=True, packing=ctx.packing)
     x2, _, _, _ = gpu_solve(c2, H, wr, 64)
     c2.close()
     assert np.array_equal(x1, x2)
+
+
+def test_packings_agree(ctx):
+    """1 and 4 shifts per wavefront run the same arithmetic (no implicit FMA
+    contraction, explicit fma()), so the vectors are bit-identical."""
+    from hsrq3in_amd import Context
+    H, wr = case(700, seed=21, m=20, kmax=200)
+    out = []
+    for pk in (1, 4):
+        c = Context(0, packing=pk)
+        out.append(gpu_solve(c, H, wr, 128)[0])
+        c.close()
+    assert np.array_equal(out[0], out[1])
 
 
 def test_batch_equals_alone(ctx):

@@ -72,6 +72,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
           file=sys.stderr, flush=True)
     total_m = sum(w.numel() for _, w, _ in items)
     plan = ctx.plan(items, tile)
+    lanes = plan.lanes()
     for _ in range(warmup):
         plan.execute(graph=True)
     plan.sync()
@@ -91,16 +92,17 @@ def batched(ctx, steps, warmup, tile, plain_sample):
         worst = max(worst, float(np.max(res / (10 * n))))
         pos += k
     finite = all(bool(torch.isfinite(v).all().item()) for _, _, v in items)
+    prof = plan.execute_profiled()
     plan.close()
     # plain path, one problem at a time (device-resident drop-in per problem)
     sample = items[::max(1, len(items) // plain_sample)]
     for Ht, wrt, vrt in sample[:2]:
-        ctx.solve(Ht, wrt, vrt, tile)
+        ctx.solve(Ht, wrt, vrt, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sm = 0
     for Ht, wrt, vrt in sample:
-        ctx.solve(Ht, wrt, vrt, tile)
+        ctx.solve(Ht, wrt, vrt, 0)   # each problem with its own automatic tile
         sm += wrt.numel()
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
@@ -108,10 +110,12 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     plain_rate = sm / plain_s
     return {"value": round(batched_rate, 1), "unit": "eigvecs/s", "problems": len(items),
             "eigvecs": total_m, "residual_ok": good, "finite": finite,
-            "worst_residual_over_bound": worst, "ms_per_batch": round(ms, 3), "tile": tile,
+            "worst_residual_over_bound": worst, "ms_per_batch": round(ms, 3),
+            "lanes": [{"tile": t, "shifts_per_wave": pk, "problems": pr} for t, pk, pr in lanes],
             "plain_one_at_a_time": {"value": round(plain_rate, 1), "problems": len(sample),
                                     "eigvecs": sm, "seconds": round(plain_s, 3)},
-            "batched_over_plain": round(batched_rate / plain_rate, 2)}
+            "batched_over_plain": round(batched_rate / plain_rate, 2),
+            "kernels": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}}
 
 
 def main():
@@ -125,9 +129,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--batch-tile", type=int, default=64)
+    ap.add_argument("--batch-tile", type=int, default=0)
     ap.add_argument("--packing", type=int, default=0)
-    ap.add_argument("--batch-packing", type=int, default=4)
+    ap.add_argument("--batch-packing", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

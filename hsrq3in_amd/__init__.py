@@ -55,9 +55,11 @@ EXPORTS = [
     "hsrq_strerror",
     "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
-    "hsrq_plan_sync", "hsrq_plan_status",
+    "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes",
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
     "hsrq_residuals_device",
+    "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
+    "hsrq_server_get_stats", "hsrq_server_destroy",
 ]
 
 _lib = None
@@ -96,6 +98,7 @@ def lib():
             getattr(L, f).argtypes = [_vp]
         L.hsrq_plan_execute_profiled.argtypes = [_vp, _dp, _ip]
         L.hsrq_plan_status.argtypes = [_vp, _ip]
+        L.hsrq_plan_lanes.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
         L.hsrq_plan_workspace_bytes.restype = ctypes.c_size_t
         L.hsrq_plan_workspace_bytes.argtypes = [_vp]
         L.hsrq_plan_destroy.restype = None
@@ -287,6 +290,15 @@ class Plan:
 
     def workspace_bytes(self) -> int:
         return int(lib().hsrq_plan_workspace_bytes(self._h))
+
+    def lanes(self) -> list:
+        """[(tile, shifts per wavefront, problems)] per lane of the plan."""
+        nl = lib().hsrq_plan_lanes(self._h, 0, None, None, None)
+        if nl < 0:
+            raise HsrqError(nl, "hsrq_plan_lanes")
+        t, pk, pr = (ctypes.c_int * nl)(), (ctypes.c_int * nl)(), (ctypes.c_int * nl)()
+        lib().hsrq_plan_lanes(self._h, nl, t, pk, pr)
+        return [(t[i], pk[i], pr[i]) for i in range(nl)]
 
     def close(self):
         if self._h:

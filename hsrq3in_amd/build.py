@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = ["hsrq_kernels.hip", "hsrq_api.cpp"]
+SOURCES = ["hsrq_kernels.hip", "hsrq_sched.cpp", "hsrq_api.cpp", "hsrq_server.cpp"]
 
 
 def _run(cmd):

@@ -1,6 +1,13 @@
-// hsrq_api.cpp -- host side of libhsrq: planner, launch schedule, restart
-// logic, C-ABI (hsrq.h) and the drop-in tiled_inverse_iteration
-// (reference inverse_iteration.h:19-22, inverse_iteration.c:128-335).
+// hsrq_api.cpp -- host runtime of libhsrq: contexts, plans (the device
+// schedule of a set of problems), restarts, the device validator, the C-ABI
+// (hsrq.h) and the drop-in tiled_inverse_iteration (reference
+// inverse_iteration.h:19-22, inverse_iteration.c:128-335).
+//
+// A plan splits its problems into lanes.  A lane is a set of problems that
+// share one tile size and one diagonal-sweep layout; its launch sequence
+// runs on its own stream, so the latency-bound sweeps of one lane overlap
+// the MFMA panel passes of another.  The lanes fork from and join into the
+// context stream, so the whole plan captures into one HIP graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,98 +21,66 @@
 
 #include "../../include/hsrq.h"
 #include "../../include/inverse_iteration.h"
-#include "hsrq_internal.h"
-
-namespace hsrq {
-hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, int, hipStream_t);
-bool norm_block_needed_host(const ProbDev& P, int rb, int cb);
-hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
-hipError_t launch_diag(int, const ProbDev*, const K1Item*, int, hipStream_t);
-hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
-hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
-hipError_t launch_diag16(int, const ProbDev*, const K1Item*, int, hipStream_t);
-hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
-hipError_t launch_mfma_selftest(int*, hipStream_t);
-hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
-                           const double* X, int64_t ldX, double* work, double* res,
-                           hipStream_t st);
-}  // namespace hsrq
+#include "hsrq_host.h"
 
 using namespace hsrq;
 
-#define HIP_TRY(expr)                         \
-  do {                                        \
-    hipError_t e_ = (expr);                   \
-    if (e_ != hipSuccess) {                   \
-      last_hip_error() = e_;                  \
-      return e_ == hipErrorOutOfMemory ? HSRQ_ENOMEM : HSRQ_EHIP; \
-    }                                         \
-  } while (0)
-
-static hipError_t& last_hip_error() {
-  static thread_local hipError_t e = hipSuccess;
-  return e;
-}
-
-struct hsrq_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int tile = 0;          // 0: choose per problem
-  int exact = 0;
-  int precision = HSRQ_PREC_FP64;
-  int lookahead = 0;     // 1: bulk panel pass on a second stream
-  int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
-};
-
 namespace {
 
-constexpr size_t kAlign = 256;
-inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+// Problems of one plan with the same tile and sweep layout, on one stream.
+struct Lane {
+  int b = 64;
+  int pack = 1;
+  std::vector<int> members;      // plan-order index of each local problem
+  std::vector<ProbDev> probs;    // host copy of the descriptors
+  ProbDev* d_probs = nullptr;
+  void* d_items = nullptr;
+  const GemmItem* norm_part = nullptr;
+  int n_norm_part = 0;
+  const RowItem* norm_fin = nullptr;
+  int n_norm_fin = 0;
+  const RowItem* init = nullptr;
+  int n_init = 0;
+  const ShiftItem* bt = nullptr;
+  int n_bt = 0;
+  int steps = 0;
+  std::vector<const K1Item*> k1, pr;
+  std::vector<int> n_k1, n_pr;
+  std::vector<const GemmItem*> ga, gb;
+  std::vector<int> n_ga, n_gb;
+  hipStream_t stream = nullptr;   // owned unless it is the context stream
+  bool own_stream = false;
+  hipStream_t side = nullptr;     // lookahead stream (context option)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
+  std::vector<hipEvent_t> ev_p, ev_b;
+};
 
-int choose_tile(const hsrq_context* ctx, int tlsz, int n) {
-  int t = tlsz > 0 ? tlsz : ctx->tile;
-  if (t <= 0) t = n >= 4096 ? 128 : 64;
-  t = ((t + kRowBlock - 1) / kRowBlock) * kRowBlock;
-  return std::min(256, std::max(64, t));
+void lane_free(Lane* L) {
+  if (!L) return;
+  for (auto e : L->ev_p) (void)hipEventDestroy(e);
+  for (auto e : L->ev_b) (void)hipEventDestroy(e);
+  if (L->ev_fork) (void)hipEventDestroy(L->ev_fork);
+  if (L->ev_join) (void)hipEventDestroy(L->ev_join);
+  if (L->ev_done) (void)hipEventDestroy(L->ev_done);
+  if (L->side) (void)hipStreamDestroy(L->side);
+  if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
+  if (L->d_probs) (void)hipFree(L->d_probs);
+  if (L->d_items) (void)hipFree(L->d_items);
+  delete L;
 }
 
 }  // namespace
 
-// A plan: device problem descriptors, workspace and the launch schedule of a
-// set of problems sharing one tile size.
 struct hsrq_plan {
   hsrq_context* ctx = nullptr;
-  int b = 64;
   int count = 0;
-  std::vector<ProbDev> probs;         // host copy
-  std::vector<int> status_off;        // prefix of m
+  std::vector<Lane*> lanes;
+  std::vector<int> lane_of, local_of;   // plan-order problem -> lane, local index
+  std::vector<int> status_off;          // prefix of m in plan order
   int total_m = 0;
-  int steps = 0;
-  // device allocations
   void* d_work = nullptr;
   size_t work_bytes = 0;
-  ProbDev* d_probs = nullptr;
-  void* d_items = nullptr;
-  // item tables (device offsets)
-  const RowItem* d_norm_items = nullptr;
-  int n_norm_items = 0;
-  const GemmItem* d_part_items = nullptr;
-  int n_part_items = 0;
-  const RowItem* d_init_items = nullptr;
-  int n_init_items = 0;
-  const ShiftItem* d_bt_items = nullptr;
-  int n_bt_items = 0;
-  std::vector<const K1Item*> k1_ptr, pr_ptr;
-  std::vector<int> k1_cnt, pr_cnt;
-  int pack = 1;   // shifts per wavefront of the diagonal sweep
-  // panel passes: rows of the next diagonal tile ("a", on the main stream,
-  // it gates the next diagonal sweep) and all rows above it ("b", on the
-  // side stream, overlapping the next diagonal sweep)
-  std::vector<const GemmItem*> ga_ptr, gb_ptr;
-  std::vector<int> ga_cnt, gb_cnt;
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  std::vector<hipEvent_t> ev_p, ev_b;
+  hipEvent_t ev_fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
@@ -116,24 +91,146 @@ void plan_free(hsrq_plan* p) {
   if (!p) return;
   if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  for (auto e : p->ev_p) (void)hipEventDestroy(e);
-  for (auto e : p->ev_b) (void)hipEventDestroy(e);
+  for (Lane* L : p->lanes) lane_free(L);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
-  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->d_work) (void)hipFree(p->d_work);
-  if (p->d_probs) (void)hipFree(p->d_probs);
-  if (p->d_items) (void)hipFree(p->d_items);
   delete p;
 }
 
-int validate_problem(const hsrq_problem& q) {
-  if (q.n <= 0 || q.m < 0 || q.ldH < q.n || q.ldvr < q.n || !q.H) return HSRQ_EINVAL;
-  if (q.m > 0 && (!q.wr || !q.vr)) return HSRQ_EINVAL;
+// Tile of the automatic policy.  A single problem below n = 4096 takes
+// 64-row tiles (half the step latency of 128); in a batch 128-row tiles win:
+// 30.3 ms vs 33.2-34.4 ms for 64-row tiles on the small problems and 44-57
+// ms with 256-row tiles (MI355X, 256 problems of n in [200, 4000]).
+int auto_tile(int n, int count) { return count == 1 && n < 4096 ? 64 : 128; }
+
+// Lane assignment: problems grouped by tile; a group with many shifts is
+// split into up to 3 lanes balanced by panel-pass work m n^2 (largest
+// first), whose launch sequences overlap on their own streams (2 lanes:
+// 30.8 ms, 3: 30.3 ms, 4: 34.3 ms against 34.7 ms for one on that batch).
+void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz,
+                  std::vector<std::vector<int>>& groups, std::vector<int>& tiles) {
+  const bool fixed = tlsz > 0 || ctx->tile > 0;
+  std::vector<int> tile_of(count);
+  int nmax = 0;
+  for (int i = 0; i < count; ++i) nmax = std::max(nmax, in[i].n);
+  for (int i = 0; i < count; ++i)
+    tile_of[i] = fixed ? choose_tile(ctx, tlsz, nmax)
+                       : choose_tile(ctx, auto_tile(in[i].n, count), 0);
+  std::vector<int> distinct(tile_of);
+  std::sort(distinct.begin(), distinct.end());
+  distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+  for (int b : distinct) {
+    std::vector<int> g;
+    long long shifts = 0;
+    for (int i = 0; i < count; ++i)
+      if (tile_of[i] == b) {
+        g.push_back(i);
+        shifts += in[i].m;
+      }
+    const int nl = count < 8 ? 1 : (int)std::min<long long>(3, std::max<long long>(1, shifts / (2 * kPackedMinShifts)));
+    if (nl == 1) {
+      groups.push_back(g);
+      tiles.push_back(b);
+      continue;
+    }
+    std::vector<int> ord(g);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) {
+      return (double)in[a].m * in[a].n * in[a].n > (double)in[c].m * in[c].n * in[c].n;
+    });
+    std::vector<std::vector<int>> h(nl);
+    std::vector<double> load(nl, 0.0);
+    for (int i : ord) {
+      const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      h[t].push_back(i);
+      load[t] += (double)in[i].m * in[i].n * in[i].n;
+    }
+    for (auto& x : h) {
+      std::sort(x.begin(), x.end());
+      if (!x.empty()) {
+        groups.push_back(x);
+        tiles.push_back(b);
+      }
+    }
+  }
+}
+
+// Item tables of one lane, uploaded once.
+int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts) {
+  L->pack = choose_pack(ctx->pack, L->b, total_lane_shifts);
+  std::vector<GemmItem> part;
+  std::vector<RowItem> fin, init;
+  std::vector<ShiftItem> bt;
+  int steps = 0;
+  const int cnt = (int)L->probs.size();
+  for (int i = 0; i < cnt; ++i) {
+    const ProbDev& P = L->probs[i];
+    if (P.m == 0) continue;
+    steps = std::max(steps, P.T);
+    add_norm_items(P, i, part, fin);
+    add_init_items(P, i, init);
+    add_bt_items(P, i, bt);
+  }
+  L->steps = steps;
+  // all problems start at their last tile column together: step s works on
+  // tile column T_i - 1 - s of every problem that still has one
+  std::vector<StepItems> st(steps);
+  for (int s = 0; s < steps; ++s)
+    for (int i = 0; i < cnt; ++i)
+      add_step_items(L->probs[i], i, L->probs[i].T - 1 - s, L->pack, st[s]);
+
+  size_t nbytes = align_up(part.size() * sizeof(GemmItem)) + align_up(fin.size() * sizeof(RowItem)) +
+                  align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem));
+  for (const auto& x : st)
+    nbytes += align_up(x.k1.size() * sizeof(K1Item)) + align_up(x.pr.size() * sizeof(K1Item)) +
+              align_up(x.ga.size() * sizeof(GemmItem)) + align_up(x.gb.size() * sizeof(GemmItem));
+  nbytes = std::max(nbytes, kAlign);
+  std::vector<char> host(nbytes);
+  HIP_TRY(hipMalloc(&L->d_items, nbytes));
+  HIP_TRY(hipMalloc(&L->d_probs, std::max<size_t>(cnt, 1) * sizeof(ProbDev)));
+  size_t pos = 0;
+  auto put = [&](const void* src, size_t sz) -> const void* {
+    if (sz) std::memcpy(host.data() + pos, src, sz);
+    const void* d = (const char*)L->d_items + pos;
+    pos += align_up(sz);
+    return d;
+  };
+  L->norm_part = (const GemmItem*)put(part.data(), part.size() * sizeof(GemmItem));
+  L->n_norm_part = (int)part.size();
+  L->norm_fin = (const RowItem*)put(fin.data(), fin.size() * sizeof(RowItem));
+  L->n_norm_fin = (int)fin.size();
+  L->init = (const RowItem*)put(init.data(), init.size() * sizeof(RowItem));
+  L->n_init = (int)init.size();
+  L->bt = (const ShiftItem*)put(bt.data(), bt.size() * sizeof(ShiftItem));
+  L->n_bt = (int)bt.size();
+  for (const auto& x : st) {
+    L->k1.push_back((const K1Item*)put(x.k1.data(), x.k1.size() * sizeof(K1Item)));
+    L->n_k1.push_back((int)x.k1.size());
+    L->pr.push_back((const K1Item*)put(x.pr.data(), x.pr.size() * sizeof(K1Item)));
+    L->n_pr.push_back((int)x.pr.size());
+    L->ga.push_back((const GemmItem*)put(x.ga.data(), x.ga.size() * sizeof(GemmItem)));
+    L->n_ga.push_back((int)x.ga.size());
+    L->gb.push_back((const GemmItem*)put(x.gb.data(), x.gb.size() * sizeof(GemmItem)));
+    L->n_gb.push_back((int)x.gb.size());
+  }
+  HIP_TRY(hipMemcpy(L->d_items, host.data(), nbytes, hipMemcpyHostToDevice));
+  if (cnt > 0)
+    HIP_TRY(hipMemcpy(L->d_probs, L->probs.data(), cnt * sizeof(ProbDev), hipMemcpyHostToDevice));
+  if (L->own_stream) HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  if (ctx->lookahead) HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
+  L->ev_p.assign(steps, nullptr);
+  L->ev_b.assign(steps, nullptr);
+  for (int s = 0; s < steps; ++s) {
+    HIP_TRY(hipEventCreateWithFlags(&L->ev_p[s], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&L->ev_b[s], hipEventDisableTiming));
+  }
   return HSRQ_OK;
 }
 
-// Builds problem descriptors, workspace and item tables.  its: start vector.
+// Builds descriptors, one workspace allocation and the lanes.  its: start
+// vector (-1: eps3 * 1; >= 0: the its-th restart vector).
 int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, int its,
                hsrq_plan** out) {
   *out = nullptr;
@@ -146,244 +243,84 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   hsrq_plan* p = new (std::nothrow) hsrq_plan();
   if (!p) return HSRQ_ENOMEM;
   p->ctx = ctx;
-  int nmax = 0;
-  for (int i = 0; i < count; ++i) nmax = std::max(nmax, in[i].n);
-  p->b = choose_tile(ctx, tlsz, nmax);
-  const int b = p->b;
-
-  // --- descriptors and workspace layout ---
-  std::vector<size_t> off;   // per problem: offset table
-  size_t bytes = 0;
-  p->probs.resize(count);
-  p->status_off.resize(count + 1, 0);
-  struct Lay {
-    size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
-  };
-  std::vector<Lay> lay(count);
-  for (int i = 0; i < count; ++i) {
-    const hsrq_problem& q = in[i];
-    ProbDev& P = p->probs[i];
-    std::memset(&P, 0, sizeof(P));
-    P.H = q.H;
-    P.ldH = q.ldH;
-    P.n = q.n;
-    P.m = q.m;
-    P.b = b;
-    P.T = (q.n + b - 1) / b;
-    P.b0 = q.n - (P.T - 1) * b;
-    P.nb0blk = (P.b0 + kRowBlock - 1) / kRowBlock;
-    P.nrb = P.nb0blk + (P.T - 1) * (b / kRowBlock);
-    P.wr = q.wr;
-    P.Y = q.vr;
-    P.ldY = q.ldvr;
-    P.its = its;
-    P.force_exact = ctx->exact;
-    const size_t nm = (size_t)q.n * q.m * sizeof(double);
-    const size_t Tm = (size_t)P.T * q.m;
-    Lay& L = lay[i];
-    L.rt0 = bytes; bytes += align_up(nm);
-    L.rt1 = bytes; bytes += align_up(nm);
-    L.c = bytes; bytes += align_up(nm);
-    L.s = bytes; bytes += align_up(nm);
-    L.bm0 = bytes; bytes += align_up((size_t)b * 2 * q.m * sizeof(double));
-    L.bm1 = bytes; bytes += align_up((size_t)b * 2 * q.m * sizeof(double));
-    L.aux0 = bytes; bytes += align_up((size_t)4 * q.m * sizeof(double));
-    L.aux1 = bytes; bytes += align_up((size_t)4 * q.m * sizeof(double));
-    L.ynb = bytes; bytes += align_up((size_t)P.nrb * q.m * sizeof(double));
-    L.rowinf = bytes; bytes += align_up((size_t)P.T * sizeof(double));
-    L.diagn = bytes; bytes += align_up((size_t)P.T * sizeof(double));
-    L.part = bytes; bytes += align_up((size_t)P.nrb * q.n * sizeof(double));
-    L.scal = bytes; bytes += align_up(4 * sizeof(double));
-    L.e = bytes; bytes += align_up(Tm * sizeof(int));
-    L.sy = bytes; bytes += align_up(Tm * sizeof(int));
-    L.sz = bytes; bytes += align_up(Tm * sizeof(int));
-    L.st = bytes; bytes += align_up((size_t)std::max(q.m, 1) * sizeof(int));
-    p->status_off[i + 1] = p->status_off[i] + q.m;
-  }
-  p->total_m = p->status_off[count];
   p->count = count;
+  p->status_off.assign(count + 1, 0);
+  for (int i = 0; i < count; ++i) p->status_off[i + 1] = p->status_off[i] + in[i].m;
+  p->total_m = p->status_off[count];
+
+  std::vector<std::vector<int>> groups;
+  std::vector<int> tiles;
+  assign_lanes(ctx, count, in, tlsz, groups, tiles);
+  p->lane_of.assign(count, 0);
+  p->local_of.assign(count, 0);
+  size_t bytes = 0;
+  std::vector<size_t> ws_off(count, 0);
+  for (size_t l = 0; l < groups.size(); ++l)
+    for (size_t j = 0; j < groups[l].size(); ++j) {
+      const int i = groups[l][j];
+      p->lane_of[i] = (int)l;
+      p->local_of[i] = (int)j;
+      ws_off[i] = bytes;
+      bytes += prob_workspace_bytes(in[i].n, in[i].m, tiles[l]);
+    }
   p->work_bytes = bytes;
-  hipError_t e = hipMalloc(&p->d_work, std::max<size_t>(bytes, kAlign));
+  hipError_t e = hipMalloc(&p->d_work, std::max(bytes, kAlign));
   if (e != hipSuccess) {
     last_hip_error() = e;
     plan_free(p);
     return HSRQ_ENOMEM;
   }
-  char* base = (char*)p->d_work;
-  for (int i = 0; i < count; ++i) {
-    ProbDev& P = p->probs[i];
-    const Lay& L = lay[i];
-    P.Rt[0] = (double*)(base + L.rt0);
-    P.Rt[1] = (double*)(base + L.rt1);
-    P.C = (double*)(base + L.c);
-    P.S = (double*)(base + L.s);
-    P.Bm[0] = (double*)(base + L.bm0);
-    P.Bm[1] = (double*)(base + L.bm1);
-    P.aux[0] = (double*)(base + L.aux0);
-    P.aux[1] = (double*)(base + L.aux1);
-    P.Ynb = (double*)(base + L.ynb);
-    P.rowinf = (double*)(base + L.rowinf);
-    P.diagn = (double*)(base + L.diagn);
-    P.part = (double*)(base + L.part);
-    P.scal = (double*)(base + L.scal);
-    P.E = (int*)(base + L.e);
-    P.SY = (int*)(base + L.sy);
-    P.SZ = (int*)(base + L.sz);
-    P.status = (int*)(base + L.st);
+  int rc = HSRQ_OK;
+  for (size_t l = 0; l < groups.size() && !rc; ++l) {
+    Lane* L = new (std::nothrow) Lane();
+    if (!L) {
+      rc = HSRQ_ENOMEM;
+      break;
+    }
+    p->lanes.push_back(L);
+    L->b = tiles[l];
+    L->members = groups[l];
+    L->own_stream = l > 0;
+    L->stream = l == 0 ? ctx->stream : nullptr;
+    int shifts = 0;
+    for (int i : groups[l]) {
+      ProbDev P;
+      prob_describe(P, in[i], L->b, its, ctx->exact, (char*)p->d_work + ws_off[i]);
+      L->probs.push_back(P);
+      shifts += in[i].m;
+    }
+    rc = lane_build(ctx, L, shifts);
   }
-
-  // --- item tables ---
-  std::vector<RowItem> norm_items, init_items;
-  std::vector<GemmItem> part_items;
-  std::vector<ShiftItem> bt_items;
-  int steps = 0;
-  for (int i = 0; i < count; ++i) {
-    const ProbDev& P = p->probs[i];
-    if (P.m == 0) continue;
-    steps = std::max(steps, P.T);
-    for (int t = 0; t < P.T; ++t) norm_items.push_back({i, t, 0, 0});
-    for (int rb = 0; rb < P.nrb; ++rb)
-      for (int cb = 0; cb < P.nrb; ++cb)
-        if (norm_block_needed_host(P, rb, cb)) part_items.push_back({i, 0, rb, cb, 1, 0});
-    for (int rb = 0; rb < P.nrb; ++rb)
-      for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init_items.push_back({i, rb, k0, 0});
-    for (int k = 0; k < P.m; ++k) bt_items.push_back({i, k, 0, 0});
-  }
-  p->steps = steps;
-  std::vector<std::vector<K1Item>> k1(steps), pr(steps);
-  // packed sweep (4 shifts per wavefront, 16 per workgroup) unless asked
-  // otherwise: it issues ~4x fewer instructions per shift step; one shift
-  // per wavefront keeps the shortest latency per step for single problems.
-  p->pack = ctx->pack ? ctx->pack : 4;
-  if (b % 64 != 0 || (b / 16 & (b / 16 - 1)) != 0) p->pack = 1;   // 16 lanes x 2^k rows
-  const int k1step = p->pack == 4 ? 16 : kShiftsPerK1;
-  std::vector<std::vector<GemmItem>> ga(steps), gb(steps);
-  for (int s = 0; s < steps; ++s) {
-    for (int i = 0; i < count; ++i) {
-      const ProbDev& P = p->probs[i];
-      if (P.m == 0) continue;
-      const int J = P.T - 1 - s;
-      if (J < 0) continue;
-      for (int k0 = 0; k0 < P.m; k0 += k1step) k1[s].push_back({i, J, k0, 0});
-      for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) pr[s].push_back({i, J, k0, 0});
-      if (J >= 1) {
-        // rows of the next diagonal tile (part a) and all rows above (part
-        // b), in pairs of row blocks; XCD-aware order: blocks are dealt to
-        // the 8 XCDs round robin, so the pairs 8t..8t+7 go out interleaved
-        // and every XCD sweeps all column blocks of one pair from its L2.
-        const int nrbJ = tile_first_rowblock(P, J);
-        const int rbn = tile_first_rowblock(P, J - 1);
-        const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
-        auto emit = [&](std::vector<GemmItem>& dst, int rs, int re) {
-          std::vector<std::pair<int, int>> pairs;
-          for (int rb = rs; rb < re; rb += kGemmRowBlocks)
-            pairs.push_back({rb, std::min(kGemmRowBlocks, re - rb)});
-          for (size_t t = 0; t < pairs.size(); t += 8) {
-            const size_t te = std::min(pairs.size(), t + 8);
-            for (int cb = 0; cb < ncb; ++cb)
-              for (size_t q = t; q < te; ++q)
-                dst.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
-          }
-        };
-        emit(ga[s], rbn, nrbJ);
-        emit(gb[s], 0, rbn);
-      }
+  if (!rc) {
+    e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      last_hip_error() = e;
+      rc = HSRQ_EHIP;
     }
   }
-  size_t nbytes = 0;
-  nbytes += align_up(norm_items.size() * sizeof(RowItem));
-  nbytes += align_up(part_items.size() * sizeof(GemmItem));
-  nbytes += align_up(init_items.size() * sizeof(RowItem));
-  nbytes += align_up(bt_items.size() * sizeof(ShiftItem));
-  for (int s = 0; s < steps; ++s) {
-    nbytes += align_up(k1[s].size() * sizeof(K1Item));
-    nbytes += align_up(pr[s].size() * sizeof(K1Item));
-    nbytes += align_up(ga[s].size() * sizeof(GemmItem));
-    nbytes += align_up(gb[s].size() * sizeof(GemmItem));
-  }
-  std::vector<char> host(std::max<size_t>(nbytes, kAlign));
-  e = hipMalloc(&p->d_items, std::max<size_t>(nbytes, kAlign));
-  if (e == hipSuccess) e = hipMalloc(&p->d_probs, std::max<size_t>(count, 1) * sizeof(ProbDev));
-  if (e != hipSuccess) {
-    last_hip_error() = e;
+  if (rc) {
     plan_free(p);
-    return HSRQ_ENOMEM;
-  }
-  size_t pos = 0;
-  auto put = [&](const void* src, size_t sz) -> const void* {
-    if (sz) std::memcpy(host.data() + pos, src, sz);
-    const void* dptr = (const char*)p->d_items + pos;
-    pos += align_up(sz);
-    return dptr;
-  };
-  p->d_norm_items = (const RowItem*)put(norm_items.data(), norm_items.size() * sizeof(RowItem));
-  p->n_norm_items = (int)norm_items.size();
-  p->d_part_items = (const GemmItem*)put(part_items.data(), part_items.size() * sizeof(GemmItem));
-  p->n_part_items = (int)part_items.size();
-  p->d_init_items = (const RowItem*)put(init_items.data(), init_items.size() * sizeof(RowItem));
-  p->n_init_items = (int)init_items.size();
-  p->d_bt_items = (const ShiftItem*)put(bt_items.data(), bt_items.size() * sizeof(ShiftItem));
-  p->n_bt_items = (int)bt_items.size();
-  p->k1_ptr.resize(steps);
-  p->k1_cnt.resize(steps);
-  p->pr_ptr.resize(steps);
-  p->pr_cnt.resize(steps);
-  p->ga_ptr.resize(steps);
-  p->ga_cnt.resize(steps);
-  p->gb_ptr.resize(steps);
-  p->gb_cnt.resize(steps);
-  for (int s = 0; s < steps; ++s) {
-    p->k1_ptr[s] = (const K1Item*)put(k1[s].data(), k1[s].size() * sizeof(K1Item));
-    p->k1_cnt[s] = (int)k1[s].size();
-    p->pr_ptr[s] = (const K1Item*)put(pr[s].data(), pr[s].size() * sizeof(K1Item));
-    p->pr_cnt[s] = (int)pr[s].size();
-    p->ga_ptr[s] = (const GemmItem*)put(ga[s].data(), ga[s].size() * sizeof(GemmItem));
-    p->ga_cnt[s] = (int)ga[s].size();
-    p->gb_ptr[s] = (const GemmItem*)put(gb[s].data(), gb[s].size() * sizeof(GemmItem));
-    p->gb_cnt[s] = (int)gb[s].size();
-  }
-  e = hipMemcpy(p->d_items, host.data(), nbytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess && count > 0)
-    e = hipMemcpy(p->d_probs, p->probs.data(), count * sizeof(ProbDev), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    last_hip_error() = e;
-    plan_free(p);
-    return HSRQ_EHIP;
-  }
-  e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
-  p->ev_p.assign(steps, nullptr);
-  p->ev_b.assign(steps, nullptr);
-  for (int s = 0; s < steps && e == hipSuccess; ++s) {
-    e = hipEventCreateWithFlags(&p->ev_p[s], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_b[s], hipEventDisableTiming);
-  }
-  if (e != hipSuccess) {
-    last_hip_error() = e;
-    plan_free(p);
-    return HSRQ_EHIP;
+    return rc;
   }
   *out = p;
   return HSRQ_OK;
 }
 
-// Launch schedule.  Main stream: norms, init, then per step s the diagonal
-// sweeps, the scaling exponents (after the previous step's bulk panel pass)
-// and the panel pass of the rows of the next diagonal tile; side stream: the
-// panel pass of all rows above, which overlaps the next diagonal sweep.
-// With `prof`, every launch is bracketed by timing events on its stream and
-// kernel_ms/launches accumulate per kind (0 norms, 1 init, 2 diag,
-// 3 update, 4 back-transform, 5 params).
+// Per-launch timing (kinds: 0 norms, 1 init, 2 diag, 3 update,
+// 4 back-transform, 5 params).
 struct Prof {
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
 };
 
-int plan_enqueue_impl(hsrq_plan* p, Prof* prof) {
-  hipStream_t st = p->ctx->stream;
-  hipStream_t sd = p->ctx->lookahead ? p->side : st;
-  const int rpl = p->b <= 64 ? 1 : (p->b <= 128 ? 2 : 4);
+// Launch sequence of one lane on its stream.  Per step: the diagonal sweeps,
+// the scaling exponents (after the previous step's panel pass of the rows
+// above), the panel pass of the next diagonal tile's rows, then the panel
+// pass of all rows above (on the lookahead stream when enabled, where it
+// overlaps the next diagonal sweep).
+int lane_enqueue(Lane* L, Prof* prof) {
+  hipStream_t st = L->stream;
+  hipStream_t sd = L->side ? L->side : st;
   auto run = [&](int kind, hipStream_t s, auto&& fn) -> int {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) {
@@ -400,50 +337,59 @@ int plan_enqueue_impl(hsrq_plan* p, Prof* prof) {
     }
     return HSRQ_OK;
   };
-  int rc;
-  rc = run(0, st, [&](hipStream_t s) {
-    return launch_norms(p->d_probs, p->d_part_items, p->n_part_items, p->d_norm_items,
-                        p->n_norm_items, s);
+  int rc = run(0, st, [&](hipStream_t s) {
+    return launch_norms(L->d_probs, L->norm_part, L->n_norm_part, L->norm_fin, L->n_norm_fin, s);
   });
   if (rc) return rc;
-  rc = run(1, st, [&](hipStream_t s) {
-    return launch_init(p->d_probs, p->d_init_items, p->n_init_items, s);
-  });
+  rc = run(1, st, [&](hipStream_t s) { return launch_init(L->d_probs, L->init, L->n_init, s); });
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(p->ev_fork, st));
-  HIP_TRY(hipStreamWaitEvent(sd, p->ev_fork, 0));
+  if (L->side) {
+    HIP_TRY(hipEventRecord(L->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(sd, L->ev_fork, 0));
+  }
   int last_b = -1;
-  for (int s = 0; s < p->steps; ++s) {
+  for (int s = 0; s < L->steps; ++s) {
     rc = run(2, st, [&](hipStream_t x) {
-      return p->pack == 4 ? launch_diag16(p->b / 16, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x)
-                          : launch_diag(rpl, p->d_probs, p->k1_ptr[s], p->k1_cnt[s], x);
+      return launch_sweep(L->pack, L->b, L->d_probs, L->k1[s], L->n_k1[s], x);
     });
     if (rc) return rc;
-    if (p->ga_cnt[s] + p->gb_cnt[s] == 0) continue;
-    if (last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, p->ev_b[last_b], 0));
-    rc = run(5, st, [&](hipStream_t x) {
-      return launch_params(p->d_probs, p->pr_ptr[s], p->pr_cnt[s], x);
-    });
+    if (L->n_ga[s] + L->n_gb[s] == 0) continue;
+    if (L->side && last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, L->ev_b[last_b], 0));
+    rc = run(5, st, [&](hipStream_t x) { return launch_params(L->d_probs, L->pr[s], L->n_pr[s], x); });
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(p->ev_p[s], st));
-    rc = run(3, st, [&](hipStream_t x) {
-      return launch_update(p->d_probs, p->ga_ptr[s], p->ga_cnt[s], x);
-    });
+    rc = run(3, st, [&](hipStream_t x) { return launch_update(L->d_probs, L->ga[s], L->n_ga[s], x); });
     if (rc) return rc;
-    HIP_TRY(hipStreamWaitEvent(sd, p->ev_p[s], 0));
-    rc = run(3, sd, [&](hipStream_t x) {
-      return launch_update(p->d_probs, p->gb_ptr[s], p->gb_cnt[s], x);
-    });
+    if (L->side) {
+      HIP_TRY(hipEventRecord(L->ev_p[s], st));
+      HIP_TRY(hipStreamWaitEvent(sd, L->ev_p[s], 0));
+    }
+    rc = run(3, sd, [&](hipStream_t x) { return launch_update(L->d_probs, L->gb[s], L->n_gb[s], x); });
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(p->ev_b[s], sd));
+    if (L->side) HIP_TRY(hipEventRecord(L->ev_b[s], sd));
     last_b = s;
   }
-  // join the side stream (also when it had no work: graph capture needs it)
-  HIP_TRY(hipEventRecord(p->ev_join, sd));
-  HIP_TRY(hipStreamWaitEvent(st, p->ev_join, 0));
-  return run(4, st, [&](hipStream_t s) {
-    return launch_backtransform(p->d_probs, p->d_bt_items, p->n_bt_items, s);
-  });
+  if (L->side) {   // join (also without work: graph capture needs it)
+    HIP_TRY(hipEventRecord(L->ev_join, sd));
+    HIP_TRY(hipStreamWaitEvent(st, L->ev_join, 0));
+  }
+  return run(4, st, [&](hipStream_t s) { return launch_backtransform(L->d_probs, L->bt, L->n_bt, s); });
+}
+
+int plan_enqueue_impl(hsrq_plan* p, Prof* prof) {
+  hipStream_t main = p->ctx->stream;
+  HIP_TRY(hipEventRecord(p->ev_fork, main));
+  for (Lane* L : p->lanes)
+    if (L->stream != main) HIP_TRY(hipStreamWaitEvent(L->stream, p->ev_fork, 0));
+  for (Lane* L : p->lanes) {
+    const int rc = lane_enqueue(L, prof);
+    if (rc) return rc;
+  }
+  for (Lane* L : p->lanes)
+    if (L->stream != main) {
+      HIP_TRY(hipEventRecord(L->ev_done, L->stream));
+      HIP_TRY(hipStreamWaitEvent(main, L->ev_done, 0));
+    }
+  return HSRQ_OK;
 }
 
 int plan_enqueue(hsrq_plan* p) { return plan_enqueue_impl(p, nullptr); }
@@ -453,7 +399,6 @@ int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
   int rc = plan_enqueue_impl(p, &prof);
   if (!rc) {
     HIP_TRY(hipStreamSynchronize(p->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(p->side));
     for (int k = 0; k < 6; ++k) {
       kernel_ms[k] = 0.0;
       launches[k] = 0;
@@ -471,7 +416,7 @@ int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
 
 int plan_read_status(hsrq_plan* p, int* status) {
   for (int i = 0; i < p->count; ++i) {
-    const ProbDev& P = p->probs[i];
+    const ProbDev& P = p->lanes[p->lane_of[i]]->probs[p->local_of[i]];
     if (P.m == 0) continue;
     HIP_TRY(hipMemcpyAsync(status + p->status_off[i], P.status, P.m * sizeof(int),
                            hipMemcpyDeviceToHost, p->ctx->stream));
@@ -480,10 +425,10 @@ int plan_read_status(hsrq_plan* p, int* status) {
   return HSRQ_OK;
 }
 
-// Solves one batch with restarts: unconverged shifts of each problem are
-// moved behind the converged ones (stable, wr and vr together) and solved
-// again with the its-th restart vector, at most n times (dlaein semantics;
-// the reference's intent at inverse_iteration.c:160-328).
+// Solves a set of problems with restarts: unconverged shifts of each problem
+// are moved behind the converged ones (stable, wr and vr together) and
+// solved again with the its-th restart vector, at most n times (dlaein
+// semantics; the reference's intent at inverse_iteration.c:160-328).
 int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz,
                         int* status_out) {
   hsrq_plan* p = nullptr;
@@ -492,8 +437,9 @@ int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs,
   rc = plan_enqueue(p);
   std::vector<int> st(std::max(p->total_m, 1));
   if (!rc) rc = plan_read_status(p, st.data());
-  const int b = p->b;
   std::vector<int> off(p->status_off);
+  std::vector<int> tile_of(count);
+  for (int i = 0; i < count; ++i) tile_of[i] = p->lanes[p->lane_of[i]]->b;
   plan_free(p);
   if (rc) return rc;
 
@@ -541,7 +487,7 @@ int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs,
       tail.wr = q.wr + first;
       tail.vr = q.vr + (size_t)first * q.ldvr;
       hsrq_plan* pt = nullptr;
-      rc = plan_build(ctx, 1, &tail, b, its, &pt);
+      rc = plan_build(ctx, 1, &tail, tile_of[i], its, &pt);
       if (rc) return rc;
       rc = plan_enqueue(pt);
       if (!rc) rc = plan_read_status(pt, sti + first);
@@ -724,6 +670,17 @@ int hsrq_plan_status(hsrq_plan* plan, int* status) {
 
 size_t hsrq_plan_workspace_bytes(const hsrq_plan* plan) { return plan ? plan->work_bytes : 0; }
 
+int hsrq_plan_lanes(const hsrq_plan* plan, int max_lanes, int* tile, int* packing, int* problems) {
+  if (!plan || max_lanes < 0) return HSRQ_EINVAL;
+  const int nl = (int)plan->lanes.size();
+  for (int l = 0; l < std::min(nl, max_lanes); ++l) {
+    if (tile) tile[l] = plan->lanes[l]->b;
+    if (packing) packing[l] = plan->lanes[l]->pack;
+    if (problems) problems[l] = (int)plan->lanes[l]->members.size();
+  }
+  return nl;
+}
+
 void hsrq_plan_destroy(hsrq_plan* plan) { plan_free(plan); }
 
 int hsrq_residuals_device(hsrq_context* ctx, int n, const double* dH, int ldH, int m,
@@ -773,7 +730,8 @@ void tiled_inverse_iteration(int n, const double* H, int ldH, int nrhs, double* 
     rc = HSRQ_EHIP;
   if (!rc && hipMemcpy(dwr, wr, (size_t)nrhs * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     rc = HSRQ_EHIP;
-  if (!rc) rc = hsrq_solve_device(ctx, n, dH, ldH, nrhs, dwr, dvr, n, tlsz, nullptr);
+  std::vector<int> status(nrhs, 0);
+  if (!rc) rc = hsrq_solve_device(ctx, n, dH, ldH, nrhs, dwr, dvr, n, tlsz, status.data());
   if (!rc && hipMemcpy2D(vr, (size_t)ldvr * sizeof(double), dvr, (size_t)n * sizeof(double),
                          (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToHost) != hipSuccess)
     rc = HSRQ_EHIP;
@@ -782,7 +740,14 @@ void tiled_inverse_iteration(int n, const double* H, int ldH, int nrhs, double* 
   if (dH) (void)hipFree(dH);
   if (dwr) (void)hipFree(dwr);
   if (dvr) (void)hipFree(dvr);
-  if (rc) std::fprintf(stderr, "tiled_inverse_iteration: %s\n", hsrq_strerror(rc));
+  if (rc) {
+    std::fprintf(stderr, "tiled_inverse_iteration: %s\n", hsrq_strerror(rc));
+    return;
+  }
+  // the reference reports each unconverged vector on stdout
+  // (shifted_Hessenberg_solve.c:232-233)
+  for (int k = 0; k < nrhs; ++k)
+    if (!(status[k] & HSRQ_CONVERGED)) std::printf("not converged\n");
 }
 
 }  // extern "C"

@@ -1,0 +1,91 @@
+// hsrq_host.h -- host-side internals shared by the plan (hsrq_api.cpp) and
+// the request server (hsrq_server.cpp): the context, error plumbing, the
+// per-problem workspace layout and descriptor, the item-table builders of
+// one sweep step, and the kernel launchers (hsrq_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "../../include/hsrq.h"
+#include "hsrq_internal.h"
+
+struct hsrq_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int tile = 0;          // 0: choose per problem
+  int exact = 0;
+  int precision = HSRQ_PREC_FP64;
+  int lookahead = 0;     // 1: bulk panel pass on a second stream
+  int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
+};
+
+namespace hsrq {
+
+hipError_t& last_hip_error();
+
+#define HIP_TRY(expr)                                                 \
+  do {                                                                \
+    hipError_t e_ = (expr);                                           \
+    if (e_ != hipSuccess) {                                           \
+      ::hsrq::last_hip_error() = e_;                                  \
+      return e_ == hipErrorOutOfMemory ? HSRQ_ENOMEM : HSRQ_EHIP;     \
+    }                                                                 \
+  } while (0)
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// Shifts in one launch from which the automatic layout packs 4 shifts per
+// wavefront (beyond 16 workgroups of 4 wavefronts per CU at 1 per wavefront).
+constexpr int kPackedMinShifts = 4096;
+
+int validate_problem(const hsrq_problem& q);
+// Tile rows for a problem of order n: explicit request, context default or
+// the size rule; rounded to a multiple of 64 in [64, 256].
+int choose_tile(const hsrq_context* ctx, int tlsz, int n);
+// Diagonal-sweep layout: 1 or 4 shifts per wavefront.
+int choose_pack(int requested, int b, int total_shifts);
+
+// Device workspace of one problem (bytes, 256-aligned pieces).
+size_t prob_workspace_bytes(int n, int m, int b);
+// Fills the descriptor of problem q with tile b, start vector `its` and the
+// workspace at `ws` (prob_workspace_bytes(q.n, q.m, b) bytes).
+void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws);
+
+// Item tables.  `i` is the problem's index in the descriptor array.
+bool norm_block_needed_host(const ProbDev& P, int rb, int cb);
+void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
+                    std::vector<RowItem>& fin);
+void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init);
+struct StepItems {
+  std::vector<K1Item> k1, pr;      // diagonal sweep; scaling exponents
+  std::vector<GemmItem> ga, gb;    // panel pass: next diagonal tile; rows above
+  void clear() {
+    k1.clear();
+    pr.clear();
+    ga.clear();
+    gb.clear();
+  }
+};
+// Items of tile column J of problem i (J = T-1 ... 0).
+void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out);
+void add_bt_items(const ProbDev& P, int i, std::vector<ShiftItem>& bt);
+
+// Launchers (hsrq_kernels.hip).  Zero counts launch nothing.
+hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, int, hipStream_t);
+hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
+hipError_t launch_diag(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
+hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
+hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
+hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
+hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
+hipError_t launch_mfma_selftest(int*, hipStream_t);
+hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
+                           const double* X, int64_t ldX, double* work, double* res,
+                           hipStream_t st);
+// Diagonal sweep of tile size b in the given layout.
+hipError_t launch_sweep(int pack, int b, const ProbDev*, const K1Item*, int, hipStream_t);
+
+}  // namespace hsrq

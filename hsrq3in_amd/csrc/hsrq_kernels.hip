@@ -1243,10 +1243,22 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 // scale, three-accumulator 2-norm, convergence test and normalisation
 // (shifted_Hessenberg_solve.c:104-245).  The rotation chain
 //   tau1_i = s_i tau1_{i-1} + c_i tau2_i,  x_{i-1} = c_i tau1_{i-1} - s_i tau2_i
-// is an affine recurrence: chunk compositions + workgroup scan.
+// is an affine recurrence.  One workgroup per shift walks the vector in
+// segments of 1024 elements: y (scaled), c and s are staged coalesced in
+// LDS, every thread composes the maps of its 4 elements, a workgroup scan
+// (carried across segments) gives each thread its entering value, and the
+// outputs leave through LDS as coalesced stores.  HBM traffic per element:
+// y, c, s read once, x written once, then read and written once more by the
+// normalisation.
 // ---------------------------------------------------------------------------
+constexpr int BT_PER = 4;              // elements per thread and segment
+constexpr int BT_SEG = 256 * BT_PER;   // elements per segment
+constexpr int BT_LDS = BT_SEG + BT_SEG / BT_PER;
+__device__ __forceinline__ int bt_pad(int e) { return e + e / BT_PER; }   // no bank conflicts
+
 __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict__ probs,
                                                        const ShiftItem* __restrict__ items) {
+  __shared__ double ly[BT_LDS], lc[BT_LDS], ls[BT_LDS], lx[BT_LDS];
   __shared__ int emin_s[4];
   __shared__ double scanA[4], scanB[4];
   __shared__ double accs[3][4];
@@ -1270,61 +1282,94 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
   double* x = P.Y + (int64_t)k * P.ldY;
   const double* c = P.C + (int64_t)k * n;
   const double* s = P.S + (int64_t)k * n;
-  const int CH = (n + 255) / 256;
-  const int i0 = min(n, tid * CH), i1 = min(n, i0 + CH);
+  const int* Ek = P.E + k;
 
-  auto alpha_of = [&](int i) -> double {
-    const int t = (i < P.b0) ? 0 : 1 + (i - P.b0) / P.b;
-    return ldexp(1.0, emin - P.E[(int64_t)t * m + k]);
-  };
-
-  // pass 1: composition of the chunk's maps
-  double A = 1.0, B = 0.0;
-  for (int i = i0; i < i1; ++i) {
-    const double t2 = alpha_of(i) * x[i];
-    const double a = (i == 0) ? 0.0 : s[i];
-    const double bb = (i == 0) ? t2 : c[i] * t2;
-    B = a * B + bb;
-    A = a * A;
-  }
-  const double ylast = (i1 > i0) ? x[i1 - 1] : 0.0;
-  // workgroup exclusive scan of the maps
-  double a = A, bb = B;
-  wave_affine_scan(a, bb);
-  if (lane == 63) {
-    scanA[wave] = a;
-    scanB[wave] = bb;
-  }
-  double ea = __shfl_up(a, 1, 64), eb = __shfl_up(bb, 1, 64);
-  if (lane == 0) {
-    ea = 1.0;
-    eb = 0.0;
-  }
-  __syncthreads();
-  double pa = 1.0, pb = 0.0;   // composite of the previous waves
-  for (int w = 0; w < wave; ++w) {
-    pb = scanA[w] * pb + scanB[w];
-    pa = scanA[w] * pa;
-  }
-  double tau1 = ea * pb + eb;   // value entering this chunk
-
-  // pass 2: the chain itself
+  double carry = 0.0;   // tau1 entering the segment
   Acc3 acc = {0.0, 0.0, 0.0};
-  for (int i = i0; i < i1; ++i) {
-    const double yi = (i == i1 - 1) ? ylast : x[i];
-    const double t2 = alpha_of(i) * yi;
-    if (i == 0) {
-      tau1 = t2;
-    } else {
-      const double xi = c[i] * tau1 - s[i] * t2;
-      tau1 = c[i] * t2 + s[i] * tau1;
-      x[i - 1] = xi;
-      acc3_add(acc, xi);
+  for (int g0 = 0; g0 < n; g0 += BT_SEG) {
+    // stage; elements beyond n are identity maps (a = 1, b = 0)
+    for (int e = tid; e < BT_SEG; e += 256) {
+      const int i = g0 + e;
+      double yv = 0.0, cv = 1.0, sv = 1.0;
+      if (i < n) {
+        const int t = (i < P.b0) ? 0 : 1 + (i - P.b0) / P.b;
+        yv = ldexp(x[i], emin - Ek[(int64_t)t * m]);
+        cv = c[i];
+        sv = s[i];
+      }
+      ly[bt_pad(e)] = yv;
+      lc[bt_pad(e)] = cv;
+      ls[bt_pad(e)] = sv;
     }
-  }
-  if (i1 == n && i1 > i0) {
-    x[n - 1] = tau1;
-    acc3_add(acc, tau1);
+    __syncthreads();
+    // composition of this thread's maps
+    const int e0 = tid * BT_PER;
+    double A = 1.0, B = 0.0;
+#pragma unroll
+    for (int j = 0; j < BT_PER; ++j) {
+      const int e = bt_pad(e0 + j);
+      const bool first = (g0 + e0 + j) == 0;
+      const double a = first ? 0.0 : ls[e];
+      const double bb = first ? ly[e] : lc[e] * ly[e];
+      B = a * B + bb;
+      A = a * A;
+    }
+    double a = A, bb = B;
+    wave_affine_scan(a, bb);
+    if (lane == 63) {
+      scanA[wave] = a;
+      scanB[wave] = bb;
+    }
+    double ea = __shfl_up(a, 1, 64), eb = __shfl_up(bb, 1, 64);
+    if (lane == 0) {
+      ea = 1.0;
+      eb = 0.0;
+    }
+    __syncthreads();
+    double pa = 1.0, pb = 0.0;   // composite of the previous waves
+    for (int w = 0; w < wave; ++w) {
+      pb = scanA[w] * pb + scanB[w];
+      pa = scanA[w] * pa;
+    }
+    double ta = 1.0, tb = 0.0;   // composite of the whole segment
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      tb = scanA[w] * tb + scanB[w];
+      ta = scanA[w] * ta;
+    }
+    double tau1 = ea * (pa * carry + pb) + eb;   // value entering this thread's chunk
+    // the chain itself
+#pragma unroll
+    for (int j = 0; j < BT_PER; ++j) {
+      const int i = g0 + e0 + j;
+      if (i < n) {
+        const int e = bt_pad(e0 + j);
+        const double t2 = ly[e];
+        if (i == 0) {
+          tau1 = t2;
+        } else {
+          const double ci = lc[e], si = ls[e];
+          const double xi = ci * tau1 - si * t2;
+          tau1 = ci * t2 + si * tau1;
+          if (e0 + j == 0)
+            x[i - 1] = xi;   // last element of the previous segment
+          else
+            lx[bt_pad(e0 + j - 1)] = xi;
+          acc3_add(acc, xi);
+        }
+        if (i == n - 1) {
+          x[n - 1] = tau1;
+          acc3_add(acc, tau1);
+        }
+      }
+    }
+    carry = ta * carry + tb;
+    __syncthreads();
+    // x_{g0+e} for e = 0 .. BT_SEG-2 was produced at element e+1
+    for (int e = tid; e < BT_SEG - 1; e += 256) {
+      const int i = g0 + e;
+      if (i + 1 < n) x[i] = lx[bt_pad(e)];
+    }
   }
   const double s0 = wave_sum(acc.asml), s1 = wave_sum(acc.amed), s2 = wave_sum(acc.abig);
   if (lane == 0) {
@@ -1523,6 +1568,12 @@ hipError_t launch_diag16(int rpl, const ProbDev* d_probs, const K1Item* items, i
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_sweep(int pack, int b, const ProbDev* d_probs, const K1Item* items, int count,
+                        hipStream_t st) {
+  if (pack == 4) return launch_diag16(b / 16, d_probs, items, count, st);
+  return launch_diag(b <= 64 ? 1 : (b <= 128 ? 2 : 4), d_probs, items, count, st);
 }
 
 hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count, hipStream_t st) {

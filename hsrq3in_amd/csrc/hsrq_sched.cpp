@@ -1,0 +1,166 @@
+// hsrq_sched.cpp -- per-problem workspace layout, descriptors and the item
+// tables of the device schedule, shared by plans and the request server.
+//
+// Tiling follows the reference's partition (partition.c:7-17): T = ceil(n/b)
+// tiles with the FIRST tile ragged, b0 = n - (T-1) b.  A problem's sweep runs
+// over tile columns J = T-1 ... 0; step J consists of the diagonal sweep of
+// tile J (all shifts), the scaling exponents of the panel pass, and the panel
+// pass over rows 0 .. l_J - 1 (two parts: the rows of tile J-1, which gate
+// the next sweep, and all rows above).
+#include <algorithm>
+#include <cstring>
+#include <utility>
+
+#include "hsrq_host.h"
+
+namespace hsrq {
+
+hipError_t& last_hip_error() {
+  static thread_local hipError_t e = hipSuccess;
+  return e;
+}
+
+int validate_problem(const hsrq_problem& q) {
+  if (q.n <= 0 || q.m < 0 || q.ldH < q.n || q.ldvr < q.n || !q.H) return HSRQ_EINVAL;
+  if (q.m > 0 && (!q.wr || !q.vr)) return HSRQ_EINVAL;
+  return HSRQ_OK;
+}
+
+int choose_tile(const hsrq_context* ctx, int tlsz, int n) {
+  int t = tlsz > 0 ? tlsz : (ctx ? ctx->tile : 0);
+  if (t <= 0) t = n >= 4096 ? 128 : 64;
+  t = ((t + kRowBlock - 1) / kRowBlock) * kRowBlock;
+  return std::min(256, std::max(64, t));
+}
+
+int choose_pack(int requested, int b, int total_shifts) {
+  // the packed sweep needs 16 lanes x 2^k rows per lane
+  if (b % 64 != 0 || ((b / 16) & (b / 16 - 1)) != 0) return 1;
+  if (requested == 1 || requested == 4) return requested;
+  return total_shifts >= kPackedMinShifts ? 4 : 1;
+}
+
+namespace {
+struct Lay {
+  size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
+  size_t total;
+};
+
+Lay layout(int n, int m, int b) {
+  const int T = (n + b - 1) / b;
+  const int b0 = n - (T - 1) * b;
+  const int nrb = (b0 + kRowBlock - 1) / kRowBlock + (T - 1) * (b / kRowBlock);
+  const size_t nm = (size_t)n * m * sizeof(double);
+  const size_t Tm = (size_t)T * m;
+  Lay L;
+  size_t x = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = x;
+    x += align_up(bytes);
+    return o;
+  };
+  L.rt0 = take(nm);
+  L.rt1 = take(nm);
+  L.c = take(nm);
+  L.s = take(nm);
+  L.bm0 = take((size_t)b * 2 * m * sizeof(double));
+  L.bm1 = take((size_t)b * 2 * m * sizeof(double));
+  L.aux0 = take((size_t)4 * m * sizeof(double));
+  L.aux1 = take((size_t)4 * m * sizeof(double));
+  L.ynb = take((size_t)nrb * m * sizeof(double));
+  L.rowinf = take((size_t)T * sizeof(double));
+  L.diagn = take((size_t)T * sizeof(double));
+  L.part = take((size_t)nrb * n * sizeof(double));
+  L.scal = take(4 * sizeof(double));
+  L.e = take(Tm * sizeof(int));
+  L.sy = take(Tm * sizeof(int));
+  L.sz = take(Tm * sizeof(int));
+  L.st = take((size_t)std::max(m, 1) * sizeof(int));
+  L.total = x;
+  return L;
+}
+}  // namespace
+
+size_t prob_workspace_bytes(int n, int m, int b) { return layout(n, m, b).total; }
+
+void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws) {
+  std::memset(&P, 0, sizeof(P));
+  P.H = q.H;
+  P.ldH = q.ldH;
+  P.n = q.n;
+  P.m = q.m;
+  P.b = b;
+  P.T = (q.n + b - 1) / b;
+  P.b0 = q.n - (P.T - 1) * b;
+  P.nb0blk = (P.b0 + kRowBlock - 1) / kRowBlock;
+  P.nrb = P.nb0blk + (P.T - 1) * (b / kRowBlock);
+  P.wr = q.wr;
+  P.Y = q.vr;
+  P.ldY = q.ldvr;
+  P.its = its;
+  P.force_exact = exact;
+  const Lay L = layout(q.n, q.m, b);
+  P.Rt[0] = (double*)(ws + L.rt0);
+  P.Rt[1] = (double*)(ws + L.rt1);
+  P.C = (double*)(ws + L.c);
+  P.S = (double*)(ws + L.s);
+  P.Bm[0] = (double*)(ws + L.bm0);
+  P.Bm[1] = (double*)(ws + L.bm1);
+  P.aux[0] = (double*)(ws + L.aux0);
+  P.aux[1] = (double*)(ws + L.aux1);
+  P.Ynb = (double*)(ws + L.ynb);
+  P.rowinf = (double*)(ws + L.rowinf);
+  P.diagn = (double*)(ws + L.diagn);
+  P.part = (double*)(ws + L.part);
+  P.scal = (double*)(ws + L.scal);
+  P.E = (int*)(ws + L.e);
+  P.SY = (int*)(ws + L.sy);
+  P.SZ = (int*)(ws + L.sz);
+  P.status = (int*)(ws + L.st);
+}
+
+void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
+                    std::vector<RowItem>& fin) {
+  if (P.m == 0) return;
+  for (int t = 0; t < P.T; ++t) fin.push_back({i, t, 0, 0});
+  for (int rb = 0; rb < P.nrb; ++rb)
+    for (int cb = 0; cb < P.nrb; ++cb)
+      if (norm_block_needed_host(P, rb, cb)) part.push_back({i, 0, rb, cb, 1, 0});
+}
+
+void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init) {
+  for (int rb = 0; rb < P.nrb; ++rb)
+    for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init.push_back({i, rb, k0, 0});
+}
+
+void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out) {
+  if (P.m == 0 || J < 0 || J >= P.T) return;
+  const int k1step = pack == 4 ? 16 : kShiftsPerK1;
+  for (int k0 = 0; k0 < P.m; k0 += k1step) out.k1.push_back({i, J, k0, 0});
+  if (J == 0) return;
+  for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) out.pr.push_back({i, J, k0, 0});
+  // Panel pass in row-block pairs, XCD-aware order: blocks are dealt to the
+  // 8 XCDs round robin, so pairs 8t .. 8t+7 go out interleaved and every XCD
+  // sweeps all column blocks of one pair from its own L2.
+  const int nrbJ = tile_first_rowblock(P, J);
+  const int rbn = tile_first_rowblock(P, J - 1);
+  const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
+  auto emit = [&](std::vector<GemmItem>& dst, int rs, int re) {
+    std::vector<std::pair<int, int>> pairs;
+    for (int rb = rs; rb < re; rb += kGemmRowBlocks)
+      pairs.push_back({rb, std::min(kGemmRowBlocks, re - rb)});
+    for (size_t t = 0; t < pairs.size(); t += 8) {
+      const size_t te = std::min(pairs.size(), t + 8);
+      for (int cb = 0; cb < ncb; ++cb)
+        for (size_t q = t; q < te; ++q) dst.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
+    }
+  };
+  emit(out.ga, rbn, nrbJ);
+  emit(out.gb, 0, rbn);
+}
+
+void add_bt_items(const ProbDev& P, int i, std::vector<ShiftItem>& bt) {
+  for (int k = 0; k < P.m; ++k) bt.push_back({i, k, 0, 0});
+}
+
+}  // namespace hsrq

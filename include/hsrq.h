@@ -103,11 +103,73 @@ int hsrq_plan_execute_profiled(hsrq_plan *plan, double *kernel_ms, int *launches
 int hsrq_plan_sync(hsrq_plan *plan);
 /* Copies the per-shift status bits (sum(m) ints) to host. */
 int hsrq_plan_status(hsrq_plan *plan, int *status);
+/* Lanes of the plan: problems sharing one tile size and diagonal-sweep
+ * layout, each lane's launch sequence on its own stream (the automatic
+ * policy picks the tile per size class and splits large lanes in two).
+ * Fills up to max_lanes entries (arrays may be NULL); returns the number of
+ * lanes or a negative code. */
+int hsrq_plan_lanes(const hsrq_plan *plan, int max_lanes, int *tile, int *packing,
+                    int *problems);
 /* Bytes of device workspace held by the plan. */
 size_t hsrq_plan_workspace_bytes(const hsrq_plan *plan);
 void hsrq_plan_destroy(hsrq_plan *plan);
 /* Stream of the context (hipStream_t), for callers that enqueue around it. */
 void *hsrq_context_stream(hsrq_context *ctx);
+
+/* ------------------------------------------------------------------------
+ * Request server: continuous batching.
+ *
+ * Requests of any n and m are submitted at any time.  A worker thread runs
+ * the device schedule in ticks; every tick advances each admitted request
+ * by one tile column of its sweep, requests of different sizes and progress
+ * sharing each launch.  A request is admitted (first come, first served) as
+ * soon as its HBM footprint fits the server's arena: workspace ~32 n m
+ * bytes, plus H, wr and X when they arrive in host memory.  H, X and the
+ * scale exponents stay in HBM until the result is fetched.  A request's
+ * results are bit-identical to those of hsrq_solve_device on the same input
+ * (same tile), alone or in any mix.
+ * ------------------------------------------------------------------------ */
+#define HSRQ_ON_HOST 0     /* H, wr, vr are host pointers (copied in/out)  */
+#define HSRQ_ON_DEVICE 1   /* H, wr, vr are device pointers, used in place */
+
+typedef struct {
+    int n;
+    const double *H;   /* n x n, column-major, ldH >= n                   */
+    int ldH;
+    int m;
+    const double *wr;  /* m shifts (device requests: reordered in place)  */
+    double *vr;        /* n x m result, ldvr >= n                          */
+    int ldvr;
+    int location;      /* HSRQ_ON_HOST or HSRQ_ON_DEVICE                   */
+} hsrq_request;
+
+typedef struct {
+    int64_t submitted, completed, eigvecs, active, queued, ticks;
+    double p50_ms, p99_ms;   /* submit-to-completion latency percentiles   */
+    double span_s;           /* first submission to last completion        */
+    size_t hbm_bytes, hbm_in_use, hbm_peak;
+} hsrq_server_stats;
+
+typedef struct hsrq_server hsrq_server;
+
+/* hbm_bytes: arena for all admitted requests; max_active: slots (admitted
+ * and not yet fetched); tlsz: tile (0: 128). */
+int hsrq_server_create(hsrq_context *ctx, size_t hbm_bytes, int max_active, int tlsz,
+                       hsrq_server **srv);
+/* Queues a request.  Host buffers are read at admission and vr is written
+ * by hsrq_server_wait: keep them valid until it returns.  HSRQ_EBUSY: the
+ * request can never fit hbm_bytes. */
+int hsrq_server_submit(hsrq_server *srv, const hsrq_request *req, int64_t *id);
+/* 1 done, 0 pending, < 0 error. */
+int hsrq_server_poll(hsrq_server *srv, int64_t id);
+/* Blocks until request id completes; copies vr to host for host requests;
+ * wr_out / status (host, m entries, may be NULL) receive the final shift
+ * order and per-shift status bits.  Releases the request's HBM. */
+int hsrq_server_wait(hsrq_server *srv, int64_t id, double *wr_out, int *status,
+                     double *latency_ms);
+int hsrq_server_get_stats(hsrq_server *srv, hsrq_server_stats *stats);
+/* Completes admitted requests, drops queued ones, frees everything. */
+void hsrq_server_destroy(hsrq_server *srv);
 
 /* Residual ||(H - wr_k I) x_k||_2 / (||H||_oo ||x_k||_2 eps), eps = 2^-52,
  * per eigenvector, computed on the device (validator).  res: host, m. */

@@ -118,6 +118,78 @@ def batched(ctx, steps, warmup, tile, plain_sample):
             "kernels": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}}
 
 
+def served(ctx, qps, nreq, pool_size, seed=9):
+    """Serving workload: Poisson arrivals at a fixed rate of requests with n
+    log-uniform in [500, 16000], m uniform in [1, n/10] shifts, 15 % graded,
+    through the continuous-batching server; H resident in HBM (a pool of
+    problems uploaded once), X and the scale exponents kept in HBM."""
+    import threading
+    import torch
+    from hsrq3in_amd import HSRQ_CONVERGED, colmajor_to_device, workloads
+    from hsrq3in_amd.serve import Server
+    t0 = time.perf_counter()
+    pool = []
+    for n, g, sd in workloads.serving_sizes(pool_size):
+        H, lam = workloads.serving_problem(n, sd, g)
+        pool.append((colmajor_to_device(H), lam, n, g))
+        del H
+    print(f"[bench] serving pool of {len(pool)} problems built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        p = int(rng.integers(len(pool)))
+        n = pool[p][2]
+        m = int(rng.integers(1, n // 10 + 1))
+        sh = rng.choice(pool[p][1], size=m, replace=False)
+        reqs.append((p, torch.from_numpy(np.ascontiguousarray(sh)).cuda(),
+                     torch.zeros((m, n), dtype=torch.float64, device="cuda")))
+    gaps = rng.exponential(1.0 / qps, size=nreq)
+    torch.cuda.synchronize()
+    srv = Server(ctx, hbm_bytes=96 << 30, max_active=512)
+    ids = [None] * nreq
+
+    def producer():
+        t_next = time.perf_counter()
+        for i, (p, wrt, vrt) in enumerate(reqs):
+            t_next += gaps[i]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit_device(pool[p][0], wrt, vrt)
+
+    th = threading.Thread(target=producer)
+    t0 = time.perf_counter()
+    th.start()
+    lat, good, total = [], 0, 0
+    for i in range(nreq):
+        while ids[i] is None:
+            time.sleep(0.0005)
+        _, wo, st, ms = srv.wait(ids[i])
+        lat.append(ms)
+        good += int(np.sum((st & HSRQ_CONVERGED) != 0))
+        total += st.size
+    wall = time.perf_counter() - t0
+    th.join()
+    stats = srv.stats()
+    srv.close()
+    # residuals of a sample of requests (device validator)
+    worst, finite = 0.0, True
+    for i in range(0, nreq, max(1, nreq // 24)):
+        p, wrt, vrt = reqs[i]
+        res = ctx.residuals(pool[p][0], wrt, vrt)
+        worst = max(worst, float(np.max(res)) / (10 * pool[p][2]))
+        finite = finite and bool(torch.isfinite(vrt).all().item())
+    return {"value": round(good / wall, 1), "unit": "eigvecs/s", "qps": qps, "requests": nreq,
+            "eigvecs": total, "converged": good, "wall_s": round(wall, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "server_p50_ms": round(stats["p50_ms"], 3), "server_p99_ms": round(stats["p99_ms"], 3),
+            "ticks": stats["ticks"], "hbm_peak_gb": round(stats["hbm_peak"] / 2**30, 3),
+            "pool": [[n, g] for _, _, n, g in pool],
+            "sample_worst_residual_over_bound": worst, "finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +204,10 @@ def main():
     ap.add_argument("--batch-tile", type=int, default=0)
     ap.add_argument("--packing", type=int, default=0)
     ap.add_argument("--batch-packing", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=100.0)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-pool", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -212,6 +288,9 @@ def main():
         ctx.set_packing(args.batch_packing)
         out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 32)
         out["batched"]["packing"] = args.batch_packing
+    if not args.no_serve:
+        ctx.set_packing(0)
+        out["served"] = served(ctx, args.serve_qps, args.serve_requests, args.serve_pool)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(H, shifts, 64, threads)

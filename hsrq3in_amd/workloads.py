@@ -137,6 +137,46 @@ def batch_sizes(count: int = 256, seed: int = 3, nmin: int = 200, nmax: int = 40
     return [(int(a), int(b), bool(c), int(d)) for a, b, c, d in zip(n, m, g, seeds)]
 
 
+# Graded range of the serving workload.  D = diag(2^k) with |k| <= 450 keeps
+# every entry 2^(k_i - k_j) H0_ij and every row sum ||H(I,:)||_oo below the
+# overflow threshold Omega = 1e300 that the robust update protection assumes
+# (reference robust.c:14-15, :244-262) up to n = 16000.  The nominal
+# k in [-600, 600] would put entries of 2^1200 (Inf) into H.
+SERVE_KMAX = 450
+
+
+def serving_problem(n: int, seed: int, graded: bool):
+    """(H, lam): an unreduced Hessenberg matrix of order n with the known real
+    spectrum lam (uniform in [-sqrt(n), sqrt(n)], random order along the
+    diagonal of B) built in O(n^2) by ``constructed_spectrum``; the
+    subdiagonal of B is ~1/sqrt(n), far below the typical distance of
+    eigenvalues adjacent on B's diagonal (~sqrt(n)), so the eigenvectors are
+    well conditioned.  Graded variants are D H D^-1 with k in
+    [-SERVE_KMAX, SERVE_KMAX] (same spectrum).  lam are used as shifts
+    unperturbed: the rounding of the construction (~1e-13 absolute) already
+    separates them from the eigenvalues of the stored H."""
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(n)
+    lam = rng.uniform(-r, r, size=n)
+    h = constructed_spectrum(lam, seed + 1, sub=1.0 / n)
+    if graded:
+        h = graded_fn(h, SERVE_KMAX, seed + 2)
+    return h, lam
+
+
+graded_fn = graded
+
+
+def serving_sizes(count: int, seed: int = 5, nmin: int = 500, nmax: int = 16000,
+                  graded_frac: float = 0.15):
+    """(n, graded, seed) of a pool of serving problems: n log-uniform in
+    [nmin, nmax], a fraction graded."""
+    rng = np.random.default_rng(seed)
+    n = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count))).astype(int)
+    g = rng.random(count) < graded_frac
+    return [(int(a), bool(b), 7000 + i) for i, (a, b) in enumerate(zip(n, g))]
+
+
 def load_shifts(name: str) -> np.ndarray:
     path = os.path.join(GOLDEN_DIR, name)
     with np.load(path, allow_pickle=False) as z:

@@ -204,7 +204,10 @@ def main():
     ap.add_argument("--batch-tile", type=int, default=0)
     ap.add_argument("--packing", type=int, default=0)
     ap.add_argument("--batch-packing", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="bytes: run one elementwise copy of this size (counter calibration)")
     ap.add_argument("--serve-qps", type=float, default=100.0)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-pool", type=int, default=12)
@@ -228,6 +231,8 @@ def main():
         shifts = workloads.real_shifts(H0, m)
     H = workloads.gaussian_hessenberg(n, seed)
     ctx = Context(0, packing=args.packing)
+    if args.lanes:
+        ctx.set_lanes(args.lanes)
     ctx.selftest()
     Ht = colmajor_to_device(H)
     wrt = torch.from_numpy(shifts.copy()).cuda()
@@ -296,6 +301,15 @@ def main():
         out["cpu_baseline"] = cpu_baseline(H, shifts, 64, threads)
     else:
         out["cpu_baseline"] = None
+    if args.copy_probe:
+        # reads and writes exactly copy_probe bytes: HBM counters of this
+        # kernel calibrate those of the solver's kernels
+        src = torch.ones(args.copy_probe // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = args.copy_probe
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0

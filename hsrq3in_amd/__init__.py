@@ -22,7 +22,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhsrq.so")
+# HSRQ_LIB: path of another build of the library (the in-tree one by default)
+LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
 
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2
@@ -51,7 +52,7 @@ EXPORTS = [
     "tiled_inverse_iteration",
     "hsrq_context_create", "hsrq_context_destroy", "hsrq_context_set_tile",
     "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_context_set_lookahead",
-    "hsrq_context_set_packing",
+    "hsrq_context_set_packing", "hsrq_context_set_lanes",
     "hsrq_strerror",
     "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
@@ -85,6 +86,7 @@ def lib():
         L.hsrq_context_set_precision.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_lookahead.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_set_packing.argtypes = [_vp, ctypes.c_int]
+        L.hsrq_context_set_lanes.argtypes = [_vp, ctypes.c_int]
         L.hsrq_context_stream.restype = _vp
         L.hsrq_context_stream.argtypes = [_vp]
         L.hsrq_selftest.argtypes = [_vp]
@@ -195,6 +197,9 @@ class Context:
     def set_packing(self, shifts_per_wave: int):
         _check(lib().hsrq_context_set_packing(self._h, shifts_per_wave), "set_packing")
         self.packing = shifts_per_wave
+
+    def set_lanes(self, lanes: int):
+        _check(lib().hsrq_context_set_lanes(self._h, lanes), "set_lanes")
 
     def set_lookahead(self, on: bool):
         _check(lib().hsrq_context_set_lookahead(self._h, int(on)), "set_lookahead")

@@ -31,7 +31,6 @@ namespace {
 struct Lane {
   int b = 64;
   int pack = 1;
-  std::vector<int> members;      // plan-order index of each local problem
   std::vector<ProbDev> probs;    // host copy of the descriptors
   ProbDev* d_probs = nullptr;
   void* d_items = nullptr;
@@ -71,11 +70,16 @@ void lane_free(Lane* L) {
 
 }  // namespace
 
+struct PieceRef {
+  int lane, local, prob, k0;
+};
+
 struct hsrq_plan {
   hsrq_context* ctx = nullptr;
   int count = 0;
   std::vector<Lane*> lanes;
-  std::vector<int> lane_of, local_of;   // plan-order problem -> lane, local index
+  std::vector<PieceRef> pieces;         // where each problem's shift ranges run
+  std::vector<int> tile_of;             // plan-order problem -> tile
   std::vector<int> status_off;          // prefix of m in plan order
   int total_m = 0;
   void* d_work = nullptr;
@@ -103,12 +107,21 @@ void plan_free(hsrq_plan* p) {
 // ms with 256-row tiles (MI355X, 256 problems of n in [200, 4000]).
 int auto_tile(int n, int count) { return count == 1 && n < 4096 ? 64 : 128; }
 
-// Lane assignment: problems grouped by tile; a group with many shifts is
-// split into up to 3 lanes balanced by panel-pass work m n^2 (largest
-// first), whose launch sequences overlap on their own streams (2 lanes:
-// 30.8 ms, 3: 30.3 ms, 4: 34.3 ms against 34.7 ms for one on that batch).
+// A contiguous range of one problem's shifts.  Shifts are independent given
+// H, so a problem can be cut into pieces that run in different lanes; every
+// shift's arithmetic is the same in any piece (results are bit-identical).
+struct Piece {
+  int prob, k0, m;
+};
+
+// Lane assignment.  Problems are grouped by tile.  Few problems (< 8): each
+// problem's shifts are cut into `nl` pieces, one per lane, so the diagonal
+// sweeps of one lane overlap the panel passes of another.  Many problems: a
+// group with many shifts is split into up to 3 lanes of whole problems,
+// balanced by panel-pass work m n^2, largest first (256-problem batch: 1
+// lane 34.7 ms, 2 lanes 30.8, 3 lanes 30.3, 4 lanes 34.3).
 void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz,
-                  std::vector<std::vector<int>>& groups, std::vector<int>& tiles) {
+                  std::vector<std::vector<Piece>>& groups, std::vector<int>& tiles) {
   const bool fixed = tlsz > 0 || ctx->tile > 0;
   std::vector<int> tile_of(count);
   int nmax = 0;
@@ -127,25 +140,40 @@ void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, in
         g.push_back(i);
         shifts += in[i].m;
       }
-    const int nl = count < 8 ? 1 : (int)std::min<long long>(3, std::max<long long>(1, shifts / (2 * kPackedMinShifts)));
-    if (nl == 1) {
-      groups.push_back(g);
-      tiles.push_back(b);
+    if (count < 8) {
+      int nl = ctx->lanes > 0 ? ctx->lanes : (shifts >= kSplitMinShifts ? 2 : 1);
+      std::vector<std::vector<Piece>> h(nl);
+      for (int i : g) {
+        // cut in multiples of 16 shifts (whole packed workgroups)
+        const int m = in[i].m, per = ((m + nl - 1) / nl + 15) / 16 * 16;
+        for (int l = 0; l < nl; ++l) {
+          const int k0 = std::min(m, l * per), k1 = std::min(m, k0 + per);
+          if (k1 > k0 || (l == 0 && m == 0)) h[l].push_back({i, k0, k1 - k0});
+        }
+      }
+      for (auto& x : h)
+        if (!x.empty()) {
+          groups.push_back(x);
+          tiles.push_back(b);
+        }
       continue;
     }
+    const int nl = ctx->lanes > 0
+                       ? ctx->lanes
+                       : (int)std::min<long long>(3, std::max<long long>(1, shifts / (2 * kPackedMinShifts)));
     std::vector<int> ord(g);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) {
       return (double)in[a].m * in[a].n * in[a].n > (double)in[c].m * in[c].n * in[c].n;
     });
-    std::vector<std::vector<int>> h(nl);
+    std::vector<std::vector<Piece>> h(nl);
     std::vector<double> load(nl, 0.0);
     for (int i : ord) {
       const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      h[t].push_back(i);
+      h[t].push_back({i, 0, in[i].m});
       load[t] += (double)in[i].m * in[i].n * in[i].n;
     }
     for (auto& x : h) {
-      std::sort(x.begin(), x.end());
+      std::sort(x.begin(), x.end(), [](const Piece& u, const Piece& v) { return u.prob < v.prob; });
       if (!x.empty()) {
         groups.push_back(x);
         tiles.push_back(b);
@@ -248,20 +276,19 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
   for (int i = 0; i < count; ++i) p->status_off[i + 1] = p->status_off[i] + in[i].m;
   p->total_m = p->status_off[count];
 
-  std::vector<std::vector<int>> groups;
+  std::vector<std::vector<Piece>> groups;
   std::vector<int> tiles;
   assign_lanes(ctx, count, in, tlsz, groups, tiles);
-  p->lane_of.assign(count, 0);
-  p->local_of.assign(count, 0);
+  p->tile_of.assign(count, 64);
   size_t bytes = 0;
-  std::vector<size_t> ws_off(count, 0);
+  std::vector<std::vector<size_t>> ws_off(groups.size());
   for (size_t l = 0; l < groups.size(); ++l)
     for (size_t j = 0; j < groups[l].size(); ++j) {
-      const int i = groups[l][j];
-      p->lane_of[i] = (int)l;
-      p->local_of[i] = (int)j;
-      ws_off[i] = bytes;
-      bytes += prob_workspace_bytes(in[i].n, in[i].m, tiles[l]);
+      const Piece& pc = groups[l][j];
+      p->pieces.push_back({(int)l, (int)j, pc.prob, pc.k0});
+      p->tile_of[pc.prob] = tiles[l];
+      ws_off[l].push_back(bytes);
+      bytes += prob_workspace_bytes(in[pc.prob].n, pc.m, tiles[l]);
     }
   p->work_bytes = bytes;
   hipError_t e = hipMalloc(&p->d_work, std::max(bytes, kAlign));
@@ -279,15 +306,19 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     }
     p->lanes.push_back(L);
     L->b = tiles[l];
-    L->members = groups[l];
     L->own_stream = l > 0;
     L->stream = l == 0 ? ctx->stream : nullptr;
     int shifts = 0;
-    for (int i : groups[l]) {
+    for (size_t j = 0; j < groups[l].size(); ++j) {
+      const Piece& pc = groups[l][j];
+      hsrq_problem q = in[pc.prob];
+      q.m = pc.m;
+      q.wr = q.wr + pc.k0;
+      q.vr = q.vr + (size_t)pc.k0 * q.ldvr;
       ProbDev P;
-      prob_describe(P, in[i], L->b, its, ctx->exact, (char*)p->d_work + ws_off[i]);
+      prob_describe(P, q, L->b, its, ctx->exact, (char*)p->d_work + ws_off[l][j]);
       L->probs.push_back(P);
-      shifts += in[i].m;
+      shifts += pc.m;
     }
     rc = lane_build(ctx, L, shifts);
   }
@@ -415,10 +446,10 @@ int plan_enqueue_profiled(hsrq_plan* p, double* kernel_ms, int* launches) {
 }
 
 int plan_read_status(hsrq_plan* p, int* status) {
-  for (int i = 0; i < p->count; ++i) {
-    const ProbDev& P = p->lanes[p->lane_of[i]]->probs[p->local_of[i]];
+  for (const PieceRef& r : p->pieces) {
+    const ProbDev& P = p->lanes[r.lane]->probs[r.local];
     if (P.m == 0) continue;
-    HIP_TRY(hipMemcpyAsync(status + p->status_off[i], P.status, P.m * sizeof(int),
+    HIP_TRY(hipMemcpyAsync(status + p->status_off[r.prob] + r.k0, P.status, P.m * sizeof(int),
                            hipMemcpyDeviceToHost, p->ctx->stream));
   }
   HIP_TRY(hipStreamSynchronize(p->ctx->stream));
@@ -438,8 +469,7 @@ int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs,
   std::vector<int> st(std::max(p->total_m, 1));
   if (!rc) rc = plan_read_status(p, st.data());
   std::vector<int> off(p->status_off);
-  std::vector<int> tile_of(count);
-  for (int i = 0; i < count; ++i) tile_of[i] = p->lanes[p->lane_of[i]]->b;
+  std::vector<int> tile_of(p->tile_of);
   plan_free(p);
   if (rc) return rc;
 
@@ -548,6 +578,12 @@ int hsrq_context_set_packing(hsrq_context* ctx, int shifts_per_wave) {
   if (!ctx || (shifts_per_wave != 0 && shifts_per_wave != 1 && shifts_per_wave != 4))
     return HSRQ_EINVAL;
   ctx->pack = shifts_per_wave;
+  return HSRQ_OK;
+}
+
+int hsrq_context_set_lanes(hsrq_context* ctx, int lanes) {
+  if (!ctx || lanes < 0 || lanes > 8) return HSRQ_EINVAL;
+  ctx->lanes = lanes;
   return HSRQ_OK;
 }
 
@@ -676,7 +712,7 @@ int hsrq_plan_lanes(const hsrq_plan* plan, int max_lanes, int* tile, int* packin
   for (int l = 0; l < std::min(nl, max_lanes); ++l) {
     if (tile) tile[l] = plan->lanes[l]->b;
     if (packing) packing[l] = plan->lanes[l]->pack;
-    if (problems) problems[l] = (int)plan->lanes[l]->members.size();
+    if (problems) problems[l] = (int)plan->lanes[l]->probs.size();
   }
   return nl;
 }

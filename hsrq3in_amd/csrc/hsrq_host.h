@@ -19,6 +19,7 @@ struct hsrq_context {
   int precision = HSRQ_PREC_FP64;
   int lookahead = 0;     // 1: bulk panel pass on a second stream
   int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
+  int lanes = 0;         // concurrent lanes of a plan (0: auto)
 };
 
 namespace hsrq {
@@ -40,6 +41,8 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 // Shifts in one launch from which the automatic layout packs 4 shifts per
 // wavefront (beyond 16 workgroups of 4 wavefronts per CU at 1 per wavefront).
 constexpr int kPackedMinShifts = 4096;
+// Shifts from which a plan of few problems cuts them into two lanes.
+constexpr int kSplitMinShifts = 1 << 30;
 
 int validate_problem(const hsrq_problem& q);
 // Tile rows for a problem of order n: explicit request, context default or

@@ -277,8 +277,8 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       rinv = 1.0 / r;
     } else {
       const double f1 = fabs(vs), g1 = fabs(hsub);
-      rare = !(f1 > 0x1p-511 && f1 < 0x1.6a09e667f3bcdp+510 && g1 > 0x1p-511 &&
-               g1 < 0x1.6a09e667f3bcdp+510);
+      rare = !((f1 > 0x1p-511) & (f1 < 0x1.6a09e667f3bcdp+510) & (g1 > 0x1p-511) &
+               (g1 < 0x1.6a09e667f3bcdp+510));
       const double d2 = fma(vs, vs, hsub * hsub);
       const double id = rsqrt_nr(d2);
       rinv = copysign(id, vs);
@@ -312,10 +312,10 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
                         double& byn) -> bool {
     const double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
     const double acsr = fabs(o.r);
-    bool rare = !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
+    bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
     const double yq = yj * o.rinv;
     byn = fma(fabs(yq), o.br, By) * slack;
-    rare = rare || !(byn <= kOmega);
+    rare = rare | !(byn <= kOmega);
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = lane + 64 * q;
@@ -683,8 +683,8 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       rinv = 1.0 / r;
     } else {
       const double f1 = fabs(vs), g1 = fabs(hsub);
-      rare = !(f1 > 0x1p-511 && f1 < 0x1.6a09e667f3bcdp+510 && g1 > 0x1p-511 &&
-               g1 < 0x1.6a09e667f3bcdp+510);
+      rare = !((f1 > 0x1p-511) & (f1 < 0x1.6a09e667f3bcdp+510) & (g1 > 0x1p-511) &
+               (g1 < 0x1.6a09e667f3bcdp+510));
       const double d2 = fma(vs, vs, hsub * hsub);
       const double id = rsqrt_nr(d2);
       rinv = copysign(id, vs);
@@ -714,10 +714,10 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
     constexpr int L = decltype(Lc)::value;
     const double yj = bc16<L>(pick<RPL>(y, st >> 4), st & 15);
     const double acsr = fabs(o.r);
-    bool rare = !(acsr >= smin) || (acsr < 1.0 && fabs(yj) > acsr * kOmega);
+    bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
     const double yq = yj * o.rinv;
     byn = fma(fabs(yq), o.br, By) * slack;
-    rare = rare || !(byn <= kOmega);
+    rare = rare | !(byn <= kOmega);
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = t + 16 * q;
@@ -801,7 +801,7 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       double byn;
       const bool rv = vstep(IC<LV>{}, s - 1, hc, false, nvo, nr, nv);
       const bool ry = ystep_fast(IC<LY>{}, s, cv, rr, ny, byn);
-      if (__any(rv || ry || exact)) return false;
+      if (__any(rv | ry | exact)) return false;
 #pragma unroll
       for (int q = 0; q < RPL; ++q) {
         y[q] = ny[q];

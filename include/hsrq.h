@@ -55,6 +55,11 @@ int hsrq_context_set_exact_norms(hsrq_context *ctx, int on);
  * shortest step latency), 4 (16 lanes per shift, ~4x fewer instructions per
  * shift step: throughput) or 0 (automatic, default). */
 int hsrq_context_set_packing(hsrq_context *ctx, int shifts_per_wave);
+/* Concurrent lanes of a plan (0: automatic, default; 1..8).  A lane is a
+ * launch sequence on its own stream; with few problems their shifts are cut
+ * into one range per lane, with many problems whole problems are dealt to
+ * lanes.  Results do not depend on the number of lanes. */
+int hsrq_context_set_lanes(hsrq_context *ctx, int lanes);
 /* 1: run the panel pass of the rows above the next diagonal tile on a
  * second stream, overlapping the next diagonal sweep (default 0). */
 int hsrq_context_set_lookahead(hsrq_context *ctx, int on);

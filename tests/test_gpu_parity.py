@@ -111,6 +111,19 @@ def test_packings_agree(ctx):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_lanes_agree(ctx, lanes):
+    """Cutting the shifts into lanes on separate streams changes nothing."""
+    from hsrq3in_amd import Context
+    H, wr = case(600, seed=77, m=70, kmax=100)
+    x1 = gpu_solve(ctx, H, wr, 128)[0]
+    c = Context(0, packing=ctx.packing)
+    c.set_lanes(lanes)
+    x2, _, st, _ = gpu_solve(c, H, wr, 128)
+    c.close()
+    assert np.array_equal(x1, x2) and (st & HSRQ_CONVERGED).all()
+
+
 def test_batch_equals_alone(ctx):
     from hsrq3in_amd import colmajor_to_device
     cases = [case(n, seed=100 + n, m=m) for n, m in [(90, 7), (257, 20), (64, 5), (500, 33)]]

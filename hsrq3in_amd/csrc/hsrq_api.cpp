@@ -727,7 +727,8 @@ int hsrq_residuals_device(hsrq_context* ctx, int n, const double* dH, int ldH, i
   HIP_TRY(hipSetDevice(ctx->device));
   double* work = nullptr;
   double* dres = nullptr;
-  HIP_TRY(hipMalloc(&work, ((size_t)n * m + n + 4) * sizeof(double)));
+  // R (n x m), row sums (n), partial row sums (64 n)
+  HIP_TRY(hipMalloc(&work, ((size_t)n * m + 65 * (size_t)n + 4) * sizeof(double)));
   hipError_t e = hipMalloc(&dres, (size_t)m * sizeof(double));
   if (e == hipSuccess)
     e = launch_residual(n, dH, ldH, m, dwr, dvr, ldvr, work, dres, ctx->stream);

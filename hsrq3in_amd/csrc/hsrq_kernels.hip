@@ -1402,21 +1402,24 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
 // column i-1 (the reference's dense product of validation.c:48-77 adds
 // zeros below the subdiagonal).
 // ---------------------------------------------------------------------------
-// r(:, k) for 16 shifts and 64 rows per workgroup: x chunks of the 16 shifts
-// are staged in LDS, every H element of the row block is read once per 16
+// r(:, k) for 64 shifts and 64 rows per workgroup: x chunks of the 64 shifts
+// are staged in LDS, every H element of the row block is read once per 64
 // shifts (columns from the first row's subdiagonal on).
+constexpr int kResShifts = 64;
 __global__ __launch_bounds__(256) void k_resid_rows(int n, const double* __restrict__ H,
                                                     int64_t ldH, int m,
                                                     const double* __restrict__ wr,
                                                     const double* __restrict__ X, int64_t ldX,
                                                     double* __restrict__ R) {
-  constexpr int RS = 16, XC = 64;
+  constexpr int RS = kResShifts, XC = 64, PER = RS / 4;
   __shared__ double xs[RS][XC];
   const int row0 = blockIdx.x * 64;
   const int k0 = blockIdx.y * RS;
-  const int rl = threadIdx.x & 63, sg = threadIdx.x >> 6;   // 4 shift groups of 4
+  const int rl = threadIdx.x & 63, sg = threadIdx.x >> 6;   // 4 shift groups of 16
   const int i = row0 + rl;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) acc[q] = 0.0;
   const int jstart = row0 > 0 ? row0 - 1 : 0;
   for (int j0 = jstart; j0 < n; j0 += XC) {
     __syncthreads();
@@ -1431,25 +1434,42 @@ __global__ __launch_bounds__(256) void k_resid_rows(int n, const double* __restr
       for (int jj = 0; jj < je; ++jj) {
         const double h = H[i + (int64_t)(j0 + jj) * ldH];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) acc[q] += h * xs[sg * 4 + q][jj];
+        for (int q = 0; q < PER; ++q) acc[q] = fma(h, xs[sg * PER + q][jj], acc[q]);
       }
     }
   }
   if (i < n) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = k0 + sg * 4 + q;
+    for (int q = 0; q < PER; ++q) {
+      const int k = k0 + sg * PER + q;
       if (k < m) R[i + (int64_t)k * n] = acc[q] - wr[k] * X[i + (int64_t)k * ldX];
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_rowsums(int n, const double* __restrict__ H, int64_t ldH,
-                                                 double* __restrict__ rs) {
+// ||H(i,:)||_1 per row in two passes: partial sums over 16 column slabs x 4
+// (64 rows x 4 column groups per workgroup, coalesced down the columns),
+// then a fixed-order sum.  Only the Hessenberg part (j >= i - 1) is read.
+constexpr int kRowSlabs = 16;
+__global__ __launch_bounds__(256) void k_rowsum_part(int n, const double* __restrict__ H,
+                                                     int64_t ldH, double* __restrict__ part) {
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int g = threadIdx.x >> 6;
+  const int slab = blockIdx.y;
+  const int w = (n + kRowSlabs - 1) / kRowSlabs;
+  const int c0 = slab * w, c1 = min(n, c0 + w);
+  double a = 0.0;
+  if (i < n)
+    for (int j = max(c0, i - 1) + g; j < c1; j += 4) a += fabs(H[i + (int64_t)j * ldH]);
+  if (i < n) part[(int64_t)(slab * 4 + g) * n + i] = a;
+}
+
+__global__ __launch_bounds__(256) void k_rowsum_final(int n, const double* __restrict__ part,
+                                                      double* __restrict__ rs) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) {
     double a = 0.0;
-    for (int j = 0; j < n; ++j) a += fabs(H[i + (int64_t)j * ldH]);
+    for (int t = 0; t < 4 * kRowSlabs; ++t) a += part[(int64_t)t * n + i];
     rs[i] = a;
   }
 }
@@ -1602,10 +1622,13 @@ hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const dou
   if (m <= 0) return hipSuccess;
   double* R = work;
   double* rs = work + (int64_t)n * m;
+  double* part = rs + n;   // 4 kRowSlabs x n
   const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_rowsums, dim3(nb), dim3(256), 0, st, n, H, ldH, rs);
-  hipLaunchKernelGGL(k_resid_rows, dim3((n + 63) / 64, (m + 15) / 16), dim3(256), 0, st, n, H, ldH,
-                     m, wr, X, ldX, R);
+  hipLaunchKernelGGL(k_rowsum_part, dim3((n + 63) / 64, kRowSlabs), dim3(256), 0, st, n, H, ldH,
+                     part);
+  hipLaunchKernelGGL(k_rowsum_final, dim3(nb), dim3(256), 0, st, n, (const double*)part, rs);
+  hipLaunchKernelGGL(k_resid_rows, dim3((n + 63) / 64, (m + kResShifts - 1) / kResShifts),
+                     dim3(256), 0, st, n, H, ldH, m, wr, X, ldX, R);
   hipLaunchKernelGGL(k_resid_norms, dim3(m), dim3(256), 0, st, n, (const double*)rs, X, ldX,
                      (const double*)R, res);
   return hipGetLastError();

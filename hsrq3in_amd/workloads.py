@@ -63,18 +63,27 @@ def graded(h0: np.ndarray, kmax: int, seed: int, frac: float = 1.0) -> np.ndarra
 
 
 def constructed_spectrum(lam: np.ndarray, seed: int, sub: float = 0.15,
-                         ratio: float = 0.9) -> np.ndarray:
+                         ratio: float = 0.9, weak=None, weak_nu: float = 0.0) -> np.ndarray:
     """Unreduced upper Hessenberg H with eigenvalues ``lam`` (O(n^2) build).
 
     B = diag(lam) + subdiag(nu) (lower bidiagonal, so eig(B) = lam exactly);
     U = diag(d) + superdiag(u), |u_j / d_{j+1}| <= ratio < 1 (cond(U) modest).
     H = U B U^-1 is computed column by column from  X U = U B.
+    ``weak``: indices whose couplings nu_{k-1}, nu_k are reduced to
+    ``weak_nu`` (times the same random factor), so that eigenvalues closer
+    together than nu (a cluster) keep well-conditioned eigenvectors.
     """
     lam = np.asarray(lam, dtype=np.float64)
     n = lam.size
     rng = np.random.default_rng(seed)
     scale = max(1.0, float(np.max(np.abs(lam)))) if n else 1.0
     nu = sub * scale * rng.uniform(0.5, 1.0, size=max(n - 1, 0))
+    if weak is not None and n > 1:
+        f = nu / (sub * scale)
+        for k in np.asarray(weak, dtype=np.int64):
+            for q in (k - 1, k):
+                if 0 <= q < n - 1:
+                    nu[q] = weak_nu * f[q]
     d = rng.uniform(1.0, 2.0, size=n)
     u = rng.uniform(-ratio, ratio, size=max(n - 1, 0)) * d[1:]
     # UB is tridiagonal: (UB)[i,i] = d_i lam_i + u_i nu_i, (UB)[i+1,i] = d_{i+1} nu_i,
@@ -99,18 +108,39 @@ def cluster_spectrum(n: int, m: int, n_cluster: int, seed: int,
     Cluster members are interleaved with well separated eigenvalues so that
     the bidiagonal B stays far from a Jordan block.  Returns (lam, shifts)
     where shifts are the m prescribed eigenvalues (cluster first) perturbed by
-    relative 1e-12.
+    relative 1e-12.  Build H with ``cluster_hessenberg``.
     """
     rng = np.random.default_rng(seed)
     lam = rng.uniform(-np.sqrt(n), np.sqrt(n), size=n)
     lam_c = 0.5 * np.sqrt(n) + 1.0
-    stride = max(1, n // max(n_cluster, 1))
-    pos = np.arange(n_cluster) * stride
+    pos = cluster_positions(n, n_cluster)
     lam[pos] = lam_c * (1.0 + spacing * np.arange(n_cluster))
     rest = np.setdiff1d(np.arange(n), pos)
     chosen = np.concatenate([pos, rng.choice(rest, size=m - n_cluster, replace=False)])
     shifts = lam[chosen] * (1.0 + SHIFT_PERTURBATION)
     return lam, shifts
+
+
+def cluster_positions(n: int, n_cluster: int) -> np.ndarray:
+    """Diagonal positions of the cluster members in ``cluster_spectrum``."""
+    stride = max(1, n // max(n_cluster, 1))
+    return np.arange(n_cluster) * stride
+
+
+def cluster_hessenberg(lam: np.ndarray, n_cluster: int, seed: int,
+                       spacing: float = 1e-8) -> np.ndarray:
+    """H with the spectrum of ``cluster_spectrum``.  B's couplings are
+    ~max|lambda|/n (far below the typical distance of eigenvalues adjacent on
+    B's diagonal, so no product of coupling/distance ratios grows) and those
+    of cluster members 0.1 x their distance (spacing |lambda_c|), so every
+    eigenvalue stays where it was put (condition numbers O(1)) and the
+    shifts are within 1e-12 of eigenvalues of the stored H."""
+    lam = np.asarray(lam)
+    n = lam.size
+    pos = cluster_positions(n, n_cluster)
+    lam_c = float(np.max(np.abs(lam[pos]))) if n_cluster else 1.0
+    return constructed_spectrum(lam, seed, sub=1.0 / max(n, 1), weak=pos,
+                                weak_nu=0.1 * spacing * lam_c)
 
 
 def real_shifts(h: np.ndarray, m: int) -> np.ndarray:

@@ -88,6 +88,22 @@ def test_graded(ctx, n, kmax, tile):
     assert d.max() <= 1e-8
 
 
+def test_cluster(ctx):
+    """20 of 50 shifts from a cluster of relative spacing 1e-8 (prescribed
+    spectrum): the cluster's eigenvectors are sensitive at ~eps ||H|| / gap,
+    so the bar is the residual and 1 - |cos| <= 1e-8 against the oracle."""
+    from hsrq3in_amd import workloads as w
+    lam, shifts = w.cluster_spectrum(900, 50, 20, 6)
+    H = w.cluster_hessenberg(lam, 20, 6)
+    x, wo, st, res = gpu_solve(ctx, H, shifts, 128)
+    xo, woo, sto = oracle.tiled_inverse_iteration(H, shifts, 128, 8)
+    assert np.isfinite(x).all()
+    assert np.array_equal(wo, woo)
+    assert (st & HSRQ_CONVERGED).all() and sto.nonconverged == 0
+    assert res.max() <= 10 * 900
+    assert one_minus_abs_cos(x, xo).max() <= 1e-8
+
+
 def test_exact_norms_identical(ctx):
     from hsrq3in_amd import Context
     H, wr = case(300, seed=11, m=12, kmax=300)

@@ -223,6 +223,17 @@ def test_residual_definition():
     assert oracle.residuals(H, wo, xe)[0].max() <= 10 * 120
 
 
+def test_cluster_hessenberg_keeps_cluster():
+    """The cluster generator's eigenvalues are where they were put (relative
+    1e-13), also inside the cluster of relative spacing 1e-8."""
+    lam, _ = w.cluster_spectrum(300, 20, 8, 3)
+    H = w.cluster_hessenberg(lam, 8, 3)
+    ev = np.linalg.eigvals(H)
+    for p in w.cluster_positions(300, 8):
+        assert np.min(np.abs(ev - lam[p])) <= 1e-13 * abs(lam[p])
+    assert np.all(np.tril(H, -2) == 0)
+
+
 def test_constructed_spectrum_known_eigenvalues():
     lam, shifts = w.cluster_spectrum(400, 30, 10, 2)
     H = w.constructed_spectrum(lam, 2)

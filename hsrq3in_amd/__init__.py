@@ -28,7 +28,8 @@ LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2
 PREC_FP64 = 0
-PREC_FP32 = 1
+PREC_FP32 = 1       # FP32 MFMA panel pass, FP64-residual check, FP64 re-solve of failures
+PREC_FP32_RAW = 2   # FP32 MFMA panel pass only
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -165,7 +166,7 @@ class Context:
     """Device, stream and settings of libhsrq (hsrq_context)."""
 
     def __init__(self, device: int = 0, tile: int = 0, exact_norms: bool = False,
-                 packing: int = 0):
+                 packing: int = 0, precision: int = 0):
         self._h = _vp()
         self.packing = packing
         _check(lib().hsrq_context_create(ctypes.byref(self._h), device), "hsrq_context_create")
@@ -175,6 +176,8 @@ class Context:
             _check(lib().hsrq_context_set_exact_norms(self._h, 1), "set_exact_norms")
         if packing:
             _check(lib().hsrq_context_set_packing(self._h, packing), "set_packing")
+        if precision:
+            _check(lib().hsrq_context_set_precision(self._h, precision), "set_precision")
 
     def close(self):
         if self._h:

@@ -31,6 +31,7 @@ namespace {
 struct Lane {
   int b = 64;
   int pack = 1;
+  int prec = HSRQ_PREC_FP64;     // panel-pass precision
   std::vector<ProbDev> probs;    // host copy of the descriptors
   ProbDev* d_probs = nullptr;
   void* d_items = nullptr;
@@ -258,7 +259,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts) {
 }
 
 // Builds descriptors, one workspace allocation and the lanes.  its: start
-// vector (-1: eps3 * 1; >= 0: the its-th restart vector).
+// vector (-1: eps3 * 1; >= 0: the its-th restart vector; -2: given in vr).
 int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, int its,
                hsrq_plan** out) {
   *out = nullptr;
@@ -306,6 +307,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
     }
     p->lanes.push_back(L);
     L->b = tiles[l];
+    L->prec = ctx->precision;
     L->own_stream = l > 0;
     L->stream = l == 0 ? ctx->stream : nullptr;
     int shifts = 0;
@@ -388,13 +390,14 @@ int lane_enqueue(Lane* L, Prof* prof) {
     if (L->side && last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, L->ev_b[last_b], 0));
     rc = run(5, st, [&](hipStream_t x) { return launch_params(L->d_probs, L->pr[s], L->n_pr[s], x); });
     if (rc) return rc;
-    rc = run(3, st, [&](hipStream_t x) { return launch_update(L->d_probs, L->ga[s], L->n_ga[s], x); });
+    auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
+    rc = run(3, st, [&](hipStream_t x) { return upd(L->d_probs, L->ga[s], L->n_ga[s], x); });
     if (rc) return rc;
     if (L->side) {
       HIP_TRY(hipEventRecord(L->ev_p[s], st));
       HIP_TRY(hipStreamWaitEvent(sd, L->ev_p[s], 0));
     }
-    rc = run(3, sd, [&](hipStream_t x) { return launch_update(L->d_probs, L->gb[s], L->n_gb[s], x); });
+    rc = run(3, sd, [&](hipStream_t x) { return upd(L->d_probs, L->gb[s], L->n_gb[s], x); });
     if (rc) return rc;
     if (L->side) HIP_TRY(hipEventRecord(L->ev_b[s], sd));
     last_b = s;
@@ -529,6 +532,196 @@ int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs,
   return HSRQ_OK;
 }
 
+// Reduced-precision solve (HSRQ_PREC_FP32): one pass with the FP32 panel
+// pass for every shift, then the FP64 residual of every vector; the shifts
+// whose vector did not converge or whose residual exceeds 10 n eps are solved
+// again on the FP64 path (with its restarts) and scattered back.  The final
+// order is the FP64 path's: converged first (stable), unchanged when every
+// shift converged.  HSRQ_PREC_FP32_RAW: the FP32 pass alone, no restarts.
+int solve_reduced(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz,
+                  int* status_out) {
+  hsrq_plan* p = nullptr;
+  int rc = plan_build(ctx, count, probs, tlsz, -1, &p);
+  if (rc) return rc;
+  rc = plan_enqueue(p);
+  std::vector<int> st(std::max(p->total_m, 1));
+  if (!rc) rc = plan_read_status(p, st.data());
+  std::vector<int> off(p->status_off);
+  std::vector<int> tile_of(p->tile_of);
+  plan_free(p);
+  if (rc) return rc;
+  if (ctx->precision == HSRQ_PREC_FP32) {
+    hipStream_t stream = ctx->stream;
+    for (int i = 0; i < count; ++i) {
+      const hsrq_problem& q = probs[i];
+      const int n = q.n, m = q.m;
+      if (m == 0) continue;
+      int* sti = st.data() + off[i];
+      // FP64 residuals of the FP32 vectors
+      double *work = nullptr, *dres = nullptr;
+      HIP_TRY(hipMalloc(&work, ((size_t)n * m + 65 * (size_t)n + 4) * sizeof(double)));
+      hipError_t e = hipMalloc(&dres, (size_t)m * sizeof(double));
+      std::vector<double> res(m);
+      if (e == hipSuccess) e = launch_residual(n, q.H, q.ldH, m, q.wr, q.vr, q.ldvr, work, dres, stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(res.data(), dres, m * sizeof(double), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      (void)hipFree(work);
+      if (dres) (void)hipFree(dres);
+      if (e != hipSuccess) {
+        last_hip_error() = e;
+        return HSRQ_EHIP;
+      }
+      std::vector<int> bad;
+      for (int k = 0; k < m; ++k)
+        if (!(sti[k] & HSRQ_CONVERGED) || !(res[k] <= 10.0 * n)) bad.push_back(k);
+      if (bad.empty()) continue;
+      // gather, solve on the FP64 path, scatter back
+      const int nb = (int)bad.size();
+      std::vector<double> wr_h(m);
+      HIP_TRY(hipMemcpy(wr_h.data(), q.wr, m * sizeof(double), hipMemcpyDeviceToHost));
+      double *twr = nullptr, *tvr = nullptr;
+      HIP_TRY(hipMalloc(&twr, (size_t)nb * sizeof(double)));
+      e = hipMalloc(&tvr, (size_t)n * nb * sizeof(double));
+      std::vector<double> wsub(nb);
+      for (int j = 0; j < nb; ++j) wsub[j] = wr_h[bad[j]];
+      if (e == hipSuccess) e = hipMemcpy(twr, wsub.data(), nb * sizeof(double), hipMemcpyHostToDevice);
+      std::vector<int> stb(nb, 0);
+      if (e == hipSuccess) {
+        const int prec = ctx->precision;
+        ctx->precision = HSRQ_PREC_FP64;
+        hsrq_problem sub = {n, q.H, q.ldH, nb, twr, tvr, n};
+        rc = solve_with_restarts(ctx, 1, &sub, tile_of[i], stb.data());
+        ctx->precision = prec;
+      } else {
+        last_hip_error() = e;
+        rc = HSRQ_EHIP;
+      }
+      if (!rc) {
+        e = hipMemcpy(wsub.data(), twr, nb * sizeof(double), hipMemcpyDeviceToHost);
+        for (int j = 0; j < nb && e == hipSuccess; ++j) {
+          e = hipMemcpyAsync(q.vr + (size_t)bad[j] * q.ldvr, tvr + (size_t)j * n,
+                             n * sizeof(double), hipMemcpyDeviceToDevice, stream);
+          wr_h[bad[j]] = wsub[j];
+          sti[bad[j]] = stb[j];
+        }
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(q.wr, wr_h.data(), m * sizeof(double), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) {
+          last_hip_error() = e;
+          rc = HSRQ_EHIP;
+        }
+      }
+      (void)hipFree(twr);
+      if (tvr) (void)hipFree(tvr);
+      if (rc) return rc;
+      // converged first, stable (a no-op when every shift converged)
+      std::vector<int> order;
+      for (int k = 0; k < m; ++k)
+        if (sti[k] & HSRQ_CONVERGED) order.push_back(k);
+      if ((int)order.size() < m) {
+        for (int k = 0; k < m; ++k)
+          if (!(sti[k] & HSRQ_CONVERGED)) order.push_back(k);
+        double* tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, (size_t)n * m * sizeof(double)));
+        std::vector<double> wr2(m);
+        std::vector<int> st2(m);
+        e = hipSuccess;
+        for (int t = 0; t < m && e == hipSuccess; ++t) {
+          wr2[t] = wr_h[order[t]];
+          st2[t] = sti[order[t]];
+          e = hipMemcpyAsync(tmp + (size_t)t * n, q.vr + (size_t)order[t] * q.ldvr,
+                             n * sizeof(double), hipMemcpyDeviceToDevice, stream);
+        }
+        if (e == hipSuccess)
+          e = hipMemcpy2DAsync(q.vr, q.ldvr * sizeof(double), tmp, n * sizeof(double),
+                               n * sizeof(double), m, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(q.wr, wr2.data(), m * sizeof(double), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) {
+          last_hip_error() = e;
+          return HSRQ_EHIP;
+        }
+        std::copy(st2.begin(), st2.end(), sti);
+      }
+    }
+  }
+  if (status_out) std::copy(st.begin(), st.begin() + off[count], status_out);
+  return HSRQ_OK;
+}
+
+// Row-norm spread max_i / min_i ||H(i,:)||_1 above which HSRQ_PREC_FP32
+// keeps a problem on the FP64 path: on graded matrices the FP32 panel pass
+// loses the small components of the vectors, and the residual relative to
+// ||H||_oo does not see it.
+constexpr double kFp32MaxRowSpread = 0x1p20;
+
+int row_spread(hsrq_context* ctx, const hsrq_problem& q, double* spread) {
+  const int n = q.n;
+  double* work = nullptr;
+  HIP_TRY(hipMalloc(&work, 65 * (size_t)n * sizeof(double)));
+  std::vector<double> rs(n);
+  hipError_t e = launch_rowsums(n, q.H, q.ldH, work + n, work, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rs.data(), work, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(work);
+  if (e != hipSuccess) {
+    last_hip_error() = e;
+    return HSRQ_EHIP;
+  }
+  double lo = INFINITY, hi = 0.0;
+  for (double v : rs) {
+    lo = std::min(lo, v);
+    hi = std::max(hi, v);
+  }
+  *spread = lo > 0.0 ? hi / lo : INFINITY;
+  return HSRQ_OK;
+}
+
+int solve_any(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz, int* status) {
+  if (ctx->precision == HSRQ_PREC_FP64)
+    return solve_with_restarts(ctx, count, probs, tlsz, status);
+  if (ctx->precision == HSRQ_PREC_FP32_RAW) return solve_reduced(ctx, count, probs, tlsz, status);
+  // HSRQ_PREC_FP32: graded problems go the FP64 way
+  std::vector<int> fp32, fp64;
+  for (int i = 0; i < count; ++i) {
+    double spread = 0.0;
+    const int rc = row_spread(ctx, probs[i], &spread);
+    if (rc) return rc;
+    (spread <= kFp32MaxRowSpread ? fp32 : fp64).push_back(i);
+  }
+  if (fp64.empty()) return solve_reduced(ctx, count, probs, tlsz, status);
+  std::vector<int> off(count + 1, 0);
+  for (int i = 0; i < count; ++i) off[i + 1] = off[i] + probs[i].m;
+  for (int pass = 0; pass < 2; ++pass) {
+    const std::vector<int>& idx = pass == 0 ? fp32 : fp64;
+    if (idx.empty()) continue;
+    std::vector<hsrq_problem> sub;
+    for (int i : idx) sub.push_back(probs[i]);
+    int tot = 0;
+    for (const auto& q : sub) tot += q.m;
+    std::vector<int> st(std::max(tot, 1));
+    const int prec = ctx->precision;
+    if (pass == 1) ctx->precision = HSRQ_PREC_FP64;
+    const int rc = pass == 0 ? solve_reduced(ctx, (int)sub.size(), sub.data(), tlsz, st.data())
+                             : solve_with_restarts(ctx, (int)sub.size(), sub.data(), tlsz, st.data());
+    ctx->precision = prec;
+    if (rc) return rc;
+    if (status) {
+      int pos = 0;
+      for (int i : idx) {
+        std::copy(st.begin() + pos, st.begin() + pos + probs[i].m, status + off[i]);
+        pos += probs[i].m;
+      }
+    }
+  }
+  return HSRQ_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -594,12 +787,10 @@ int hsrq_context_set_lookahead(hsrq_context* ctx, int on) {
 }
 
 int hsrq_context_set_precision(hsrq_context* ctx, int prec) {
-  if (!ctx) return HSRQ_EINVAL;
-  if (prec == HSRQ_PREC_FP64) {
-    ctx->precision = prec;
-    return HSRQ_OK;
-  }
-  return prec == HSRQ_PREC_FP32 ? HSRQ_EUNSUPPORTED : HSRQ_EINVAL;
+  if (!ctx || (prec != HSRQ_PREC_FP64 && prec != HSRQ_PREC_FP32 && prec != HSRQ_PREC_FP32_RAW))
+    return HSRQ_EINVAL;
+  ctx->precision = prec;
+  return HSRQ_OK;
 }
 
 void* hsrq_context_stream(hsrq_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
@@ -643,7 +834,7 @@ int hsrq_solve_device(hsrq_context* ctx, int n, const double* dH, int ldH, int m
   if (validate_problem(q)) return HSRQ_EINVAL;
   if (m == 0) return HSRQ_OK;
   HIP_TRY(hipSetDevice(ctx->device));
-  return solve_with_restarts(ctx, 1, &q, tlsz, status);
+  return solve_any(ctx, 1, &q, tlsz, status);
 }
 
 int hsrq_solve_batch_device(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz,
@@ -651,7 +842,7 @@ int hsrq_solve_batch_device(hsrq_context* ctx, int count, const hsrq_problem* pr
   if (!ctx || count < 0 || (count > 0 && !probs)) return HSRQ_EINVAL;
   if (count == 0) return HSRQ_OK;
   HIP_TRY(hipSetDevice(ctx->device));
-  return solve_with_restarts(ctx, count, probs, tlsz, status);
+  return solve_any(ctx, count, probs, tlsz, status);
 }
 
 int hsrq_plan_create(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz,

@@ -82,12 +82,16 @@ hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
+hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t);
 hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);
 hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
                            const double* X, int64_t ldX, double* work, double* res,
                            hipStream_t st);
+// ||H(i,:)||_1 per row into rs (n); work: 64 n doubles.
+hipError_t launch_rowsums(int n, const double* H, int64_t ldH, double* work, double* rs,
+                          hipStream_t st);
 // Diagonal sweep of tile size b in the given layout.
 hipError_t launch_sweep(int pack, int b, const ProbDev*, const K1Item*, int, hipStream_t);
 

@@ -42,7 +42,8 @@ struct ProbDev {
   double* part;       // partial absolute row sums, nrb x n
   double* scal;       // [0] = ||H||_oo, [1] = eps3
   int* status;        // per shift: bit0 converged, bit1 perturbed pivot
-  int its;            // -1: start vector eps3*1; >= 0: dlaein restart vector
+  int its;            // -1: start vector eps3*1; >= 0: dlaein restart vector;
+                      // -2: the right-hand sides are given in Y
   int force_exact;    // 1: always use exact norms in the diagonal sweep
 };
 

@@ -26,6 +26,7 @@
 namespace hsrq {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // K0: norms of H.  rowinf[I] = ||H(I, :)||_oo (the reference's
@@ -131,14 +132,16 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
   double yv = 0.0;
   if (lane < nr) {
     const int row = r0 + lane;
-    if (P.its < 0) {
+    if (P.its == -2) {
+      yv = P.Y[row + (int64_t)k * P.ldY];   // given right-hand side
+    } else if (P.its < 0) {
       yv = eps3;
     } else {
       const double sq = sqrt((double)n);
       yv = (row == 0) ? eps3 : eps3 / (sq + 1.0);
       if (row == n - 1 - P.its) yv -= eps3 * sq;
     }
-    P.Y[row + (int64_t)k * P.ldY] = yv;
+    if (P.its != -2) P.Y[row + (int64_t)k * P.ldY] = yv;
     double h = P.H[row + (int64_t)(n - 1) * P.ldH];
     if (row == n - 1) h -= P.wr[k];
     P.Rt[(P.T - 1) & 1][row + (int64_t)k * n] = h;
@@ -1239,6 +1242,206 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 }
 
 // ---------------------------------------------------------------------------
+// Panel pass with FP32 MFMA (v_mfma_f32_16x16x4_f32) and FP32 accumulation:
+// the optional reduced-precision update path.  Same blocking and epilogue as
+// k_update; H is staged as float scaled by 2^-eH (||H||_oo <= 2^(eH+1), so
+// the scaled entries are at most 2), [W | Z] as float (|W| <= 1, Z carries
+// its prescale), and the FP32 result is scaled back and finished in FP64
+// (cross-over terms, shift correction, scaled update of y).  FP32 MFMA
+// layout: D[i][j] in register i%4 of lane j + 16 (i/4); here D = D^T[col][row],
+// so a lane holds one row and four adjacent columns, W (even) and Z (odd).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ probs,
+                                                  const GemmItem* __restrict__ items) {
+  __shared__ float As[2][KC * LDA];
+  __shared__ float Bs[2][KC * LDB];
+  __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
+  __shared__ int psy[GN / 2], psz[GN / 2], pez[GN / 2];
+  __shared__ double red[2][GN];
+
+  const GemmItem it = items[blockIdx.x];
+  const ProbDev& P = probs[it.prob];
+  const int J = it.J;
+  const int m = P.m, n = P.n, b = P.b;
+  const int l = tile_first_row(P, J);
+  int tile, r0, nr;
+  rowblock_range(P, it.rb, tile, r0, nr);
+  const int col0 = it.cb * GN;
+  const int ncols = min(GN, 2 * m - col0);
+  const int eH = P.scal[0] > 0.0 ? ilogb(P.scal[0]) : 0;
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t ldH = P.ldH;
+
+  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;
+  const double* __restrict__ Bg = P.Bm[J & 1];
+
+  constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
+  float pa[NPA], pb[NPB];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int i = 0; i < NPA; ++i) {
+      const int idx = tid + 256 * i;
+      const int kk = idx / GM, rr = idx - kk * GM;
+      pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH], -eH) : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) {
+      const int idx = tid + 256 * i;
+      const int cc = idx / KC, k2 = idx - cc * KC;
+      // Z columns (odd) scaled to max |Z| in [1, 2) by their own power of two
+      const int ez = (cc & 1) ? pez[cc >> 1] : 0;
+      pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b], -ez) : 0.0f;
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NPA; ++i) {
+      const int idx = tid + 256 * i;
+      const int kk = idx / GM, rr = idx - kk * GM;
+      As[buf][kk * LDA + rr] = pa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NPB; ++i) {
+      const int idx = tid + 256 * i;
+      const int cc = idx / KC, k2 = idx - cc * KC;
+      Bs[buf][k2 * LDB + cc] = pb[i];
+    }
+  };
+
+  float4_t acc[2][CT];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[a][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
+
+  const int nchunks = b / KC;
+  if (tid < GN / 2) {
+    const int k = col0 / 2 + tid;
+    int ez = 0;
+    if (k < m) {
+      const double* aux = P.aux[J & 1];
+      pmu[tid] = P.wr[k];
+      pP[tid] = aux[4 * k + 0];
+      pZl[tid] = aux[4 * k + 1];
+      pb0w[tid] = Bg[(int64_t)(2 * k) * b];
+      pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
+      psy[tid] = P.SY[(int64_t)tile * m + k];
+      psz[tid] = P.SZ[(int64_t)tile * m + k];
+      const double zn = aux[4 * k + 2];   // ||Z|| before the prescale 2^-p
+      if (zn > 0.0 && isfinite(zn)) ez = ilogb(zn) - (int)aux[4 * k + 3];
+    }
+    pez[tid] = ez;
+  }
+  __syncthreads();
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kr = k4 + (lane >> 4);
+      float hv[2], zv[CT];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+        zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(zv[c], hv[a], acc[a][c], 0, 0, 0);
+    }
+    if (ch + 1 < nchunks) stash(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue.  acc[a][c][r]: row wm*32 + a*16 + lane%16,
+  //                          col wn*GN/2 + c*16 + 4*(lane/16) + r.
+  const double* __restrict__ Rin = P.Rt[J & 1];
+  double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+  const int g = lane >> 4;
+  const int rl = lane & 15;
+  double rin[CT][4][2], yv[CT][4][2];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
+      const int k = (col0 + cl) >> 1;
+      const bool okc = cl < ncols;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int row = wm * 32 + a * 16 + rl;
+        const int gi = r0 + row;
+        const bool ok = okc && row < nr;
+        rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+        yv[c][r][a] = (ok && (r & 1)) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+      }
+    }
+  double cmaxs[CT][2];
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) cmaxs[c][h] = 0.0;
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const bool isz = (r & 1) != 0;
+      const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
+      const int ks = cl >> 1;
+      const int k = (col0 + cl) >> 1;
+      const bool okc = cl < ncols;
+      if (okc) {
+        const double mu = pmu[ks];
+        const double coef = isz ? pZl[ks] : pP[ks];
+        const double b0 = isz ? pb0z[ks] : pb0w[ks];
+        const int sy = psy[ks], sz = psz[ks];
+        const int es = eH + (isz ? pez[ks] : 0);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          const int row = wm * 32 + a * 16 + rl;
+          const int gi = r0 + row;
+          if (row < nr) {
+            double val = fma(coef, rin[c][r][a], ldexp((double)acc[a][c][r], es));
+            if (gi == l - 1) val -= mu * b0;
+            if (!isz) {
+              Rout[gi + (int64_t)k * n] = val;
+            } else {
+              const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
+              P.Y[gi + (int64_t)k * P.ldY] = ynew;
+              cmaxs[c][r >> 1] = fmax(cmaxs[c][r >> 1], fabs(ynew));
+            }
+          }
+        }
+      }
+    }
+  // per-column maxima of |y| over the block's 64 rows (Z columns only)
+#pragma unroll
+  for (int c = 0; c < CT; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double x = row16_max(cmaxs[c][h]);
+      const int cl = wn * (GN / 2) + c * 16 + 4 * g + 2 * h + 1;
+      if (rl == 0) red[wm][cl] = x;
+    }
+  __syncthreads();
+  if (tid < GN) {
+    const int cl = tid;
+    if (cl < ncols && (cl & 1)) {
+      const int k = (col0 + cl) >> 1;
+      P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K5: back-transform x = Q^T y with tiles brought to the common (smallest)
 // scale, three-accumulator 2-norm, convergence test and normalisation
 // (shifted_Hessenberg_solve.c:104-245).  The rotation chain
@@ -1394,6 +1597,7 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
   if (ynrm > 0.0)
     for (int i = tid; i < n; i += 256) x[i] = x[i] / ynrm;
 }
+
 
 // ---------------------------------------------------------------------------
 // Validator (test / benchmark only): r_k = (H - wr_k I) x_k for every
@@ -1609,10 +1813,27 @@ hipError_t launch_update(const ProbDev* d_probs, const GemmItem* items, int coun
   return hipGetLastError();
 }
 
+hipError_t launch_update32(const ProbDev* d_probs, const GemmItem* items, int count,
+                           hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update32, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
 hipError_t launch_backtransform(const ProbDev* d_probs, const ShiftItem* items, int count,
                                 hipStream_t st) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_backtransform, dim3(count), dim3(256), 0, st, d_probs, items);
+  return hipGetLastError();
+}
+
+hipError_t launch_rowsums(int n, const double* H, int64_t ldH, double* work, double* rs,
+                          hipStream_t st) {
+  double* part = work;   // 4 kRowSlabs x n
+  hipLaunchKernelGGL(k_rowsum_part, dim3((n + 63) / 64, kRowSlabs), dim3(256), 0, st, n, H, ldH,
+                     part);
+  hipLaunchKernelGGL(k_rowsum_final, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                     (const double*)part, rs);
   return hipGetLastError();
 }
 

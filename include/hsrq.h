@@ -40,7 +40,9 @@ enum {
 
 /* Update precision of the panel pass. */
 #define HSRQ_PREC_FP64 0
-#define HSRQ_PREC_FP32 1   /* FP32 MFMA panel GEMM + one FP64 correction   */
+#define HSRQ_PREC_FP32 1     /* FP32 MFMA panel GEMM, FP64-residual check,
+                                FP64 re-solve of the vectors that fail it */
+#define HSRQ_PREC_FP32_RAW 2 /* FP32 MFMA panel GEMM, no correction        */
 
 typedef struct hsrq_context hsrq_context;
 
@@ -63,7 +65,16 @@ int hsrq_context_set_lanes(hsrq_context *ctx, int lanes);
 /* 1: run the panel pass of the rows above the next diagonal tile on a
  * second stream, overlapping the next diagonal sweep (default 0). */
 int hsrq_context_set_lookahead(hsrq_context *ctx, int on);
-/* Panel-pass precision, HSRQ_PREC_FP64 (default) or HSRQ_PREC_FP32. */
+/* Panel-pass precision: HSRQ_PREC_FP64 (default), HSRQ_PREC_FP32 or
+ * HSRQ_PREC_FP32_RAW.  In the FP32 modes the panel GEMM runs in FP32 on MFMA
+ * with FP32 accumulation; the diagonal sweeps, bounds, scale exponents,
+ * back-transform and residuals stay FP64.  HSRQ_PREC_FP32 (hsrq_solve_device,
+ * hsrq_solve_batch_device): after the FP32 pass the FP64 residual of every
+ * vector is computed, and the shifts whose vector did not converge or whose
+ * residual exceeds 10 n eps are solved again on the FP64 path.  Graded
+ * problems (row 1-norms spread over more than 2^20) stay on the FP64 path:
+ * there the FP32 pass loses the small vector components unseen by the
+ * residual.  HSRQ_PREC_FP32_RAW and plans: the FP32 pass alone. */
 int hsrq_context_set_precision(hsrq_context *ctx, int prec);
 const char *hsrq_strerror(int code);
 /* Checks the FP64 MFMA operand layout the kernels assume. */

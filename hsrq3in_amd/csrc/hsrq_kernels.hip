@@ -1060,14 +1060,21 @@ constexpr int GM = kRowBlock;   // 64 rows per block (one row block)
 constexpr int GN = kGemmCols;   // 64 columns per block (32 shifts)
 constexpr int LDA = GM + 4;
 constexpr int LDB = GN + 4;
+// FP64 tiles, laid out for 64 LDS banks of 4 bytes (ds_read_b64 served per
+// 32 lanes): H tile k-major with ld 80 doubles, so the four k rows read by
+// one MFMA step fall into alternating bank halves; [W | Z] tile column-major
+// with ld 34 doubles, so a column's 32 k values are written contiguously
+// and the 16 columns x 2 k read by 32 lanes hit 64 distinct banks.
+constexpr int LDA64 = GM + 16;
+constexpr int LDB64 = KC + 2;
 constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 
 // Block = one 64-row block x 64 columns of [W | Z] (32 shifts); four waves,
 // 32 rows x 32 columns each (2 x 2 MFMA tiles).
 __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
                                                 const GemmItem* __restrict__ items) {
-  __shared__ double As[2][KC * LDA];   // As[buf][k * LDA + row]
-  __shared__ double Bs[2][KC * LDB];   // Bs[buf][k * LDB + col]
+  __shared__ double As[2][KC * LDA64];   // As[buf][k * LDA64 + row]
+  __shared__ double Bs[2][GN * LDB64];   // Bs[buf][col * LDB64 + k]
   // epilogue parameters of the block's shifts (loaded during the K loop)
   __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
   __shared__ int psy[GN / 2], psz[GN / 2];
@@ -1113,13 +1120,13 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     for (int i = 0; i < NPA; ++i) {
       const int idx = tid + 256 * i;
       const int kk = idx / GM, rr = idx - kk * GM;
-      As[buf][kk * LDA + rr] = pa[i];
+      As[buf][kk * LDA64 + rr] = pa[i];
     }
 #pragma unroll
     for (int i = 0; i < NPB; ++i) {
       const int idx = tid + 256 * i;
       const int cc = idx / KC, k2 = idx - cc * KC;
-      Bs[buf][k2 * LDB + cc] = pb[i];
+      Bs[buf][cc * LDB64 + k2] = pb[i];
     }
   };
 
@@ -1149,54 +1156,63 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   }
   stash(0);
   __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = ch & 1;
-    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
-#pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      const int kr = k4 + (lane >> 4);
-      double hv[2], zv[CT];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
-#pragma unroll
-      for (int c = 0; c < CT; ++c)
-        zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
-    }
-    if (ch + 1 < nchunks) stash(buf ^ 1);
-    __syncthreads();
-  }
 
-  // Epilogue.  acc[a][c][r]: row  wm*32 + a*16 + lane%16,
-  //                          col  wn*GN/2 + c*16 + 4*r + lane/16.
-  // Even lane groups hold W columns (R~ out), odd ones Z columns (Y); all
-  // row loads are issued before any use.
+  // Epilogue operands (R~ in, y), loaded while the last K chunk runs on
+  // MFMA: acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
+  // wn*GN/2 + c*16 + 4*r + lane/16; even lane groups hold W columns (R~
+  // out), odd ones Z columns (y).
   const double* __restrict__ Rin = P.Rt[J & 1];
   double* __restrict__ Rout = P.Rt[(J + 1) & 1];
   const int g = lane >> 4;
   const bool isz = (g & 1) != 0;
   const int rl = lane & 15;
   double rin[CT][4][2], yv[CT][4][2];
+  auto load_epilogue = [&]() {
 #pragma unroll
-  for (int c = 0; c < CT; ++c)
+    for (int c = 0; c < CT; ++c)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
-      const int k = (col0 + cl) >> 1;
-      const bool okc = cl < ncols;
+      for (int r = 0; r < 4; ++r) {
+        const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
+        const int k = (col0 + cl) >> 1;
+        const bool okc = cl < ncols;
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int row = wm * 32 + a * 16 + rl;
-        const int gi = r0 + row;
-        const bool ok = okc && row < nr;
-        rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-        yv[c][r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+        for (int a = 0; a < 2; ++a) {
+          const int row = wm * 32 + a * 16 + rl;
+          const int gi = r0 + row;
+          const bool ok = okc && row < nr;
+          rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+          yv[c][r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+        }
       }
+  };
+  auto mfma_chunk = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k4 = 0; k4 < KC; k4 += 4) {
+      const int kr = k4 + (lane >> 4);
+      double hv[2], zv[CT];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA64 + wm * 32 + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int c = 0; c < CT; ++c)
+        zv[c] = Bs[buf][(wn * (GN / 2) + c * 16 + (lane & 15)) * LDB64 + kr];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
     }
+  };
+  for (int ch = 0; ch + 1 < nchunks; ++ch) {
+    const int buf = ch & 1;
+    fetch((ch + 1) * KC);
+    mfma_chunk(buf);
+    stash(buf ^ 1);
+    __syncthreads();
+  }
+  load_epilogue();   // the staging registers are free now
+  mfma_chunk((nchunks - 1) & 1);
+
+  // Epilogue.
 #pragma unroll
   for (int c = 0; c < CT; ++c)
 #pragma unroll

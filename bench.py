@@ -76,6 +76,12 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     print(f"[bench] batch of {len(items)} problems built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     total_m = sum(w.numel() for _, w, _ in items)
+    tuned = None
+    if tile == 0:
+        t0 = time.perf_counter()
+        tuned = ctx.autotune(items)
+        tuned["tuning_s"] = round(time.perf_counter() - t0, 2)
+        print(f"[bench] autotuned batch: {tuned}", file=sys.stderr, flush=True)
     plan = ctx.plan(items, tile)
     lanes = plan.lanes()
     for _ in range(warmup):
@@ -99,7 +105,9 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     finite = all(bool(torch.isfinite(v).all().item()) for _, _, v in items)
     prof = plan.execute_profiled()
     plan.close()
-    # plain path, one problem at a time (device-resident drop-in per problem)
+    # plain path, one problem at a time (device-resident drop-in per problem),
+    # each with the automatic per-problem choices
+    ctx.reset_tuning()
     sample = items[::max(1, len(items) // plain_sample)]
     for Ht, wrt, vrt in sample[:2]:
         ctx.solve(Ht, wrt, vrt, 0)
@@ -117,6 +125,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
             "eigvecs": total_m, "residual_ok": good, "finite": finite,
             "worst_residual_over_bound": worst, "ms_per_batch": round(ms, 3),
             "lanes": [{"tile": t, "shifts_per_wave": pk, "problems": pr} for t, pk, pr in lanes],
+            "autotuned": tuned,
             "plain_one_at_a_time": {"value": round(plain_rate, 1), "problems": len(sample),
                                     "eigvecs": sm, "seconds": round(plain_s, 3)},
             "batched_over_plain": round(batched_rate / plain_rate, 2),
@@ -410,7 +419,7 @@ def main():
     if args.precision_study:
         out["precision"] = {"graded": precision_study(), "ungraded": precision_study(frac=0.0)}
     if not args.no_serve:
-        ctx.set_packing(0)
+        ctx.reset_tuning()
         out["served"] = served(ctx, args.serve_qps, args.serve_requests, args.serve_pool)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -57,7 +57,7 @@ EXPORTS = [
     "hsrq_strerror",
     "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
-    "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes",
+    "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes", "hsrq_autotune",
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
     "hsrq_residuals_device",
     "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
@@ -102,6 +102,7 @@ def lib():
         L.hsrq_plan_execute_profiled.argtypes = [_vp, _dp, _ip]
         L.hsrq_plan_status.argtypes = [_vp, _ip]
         L.hsrq_plan_lanes.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
+        L.hsrq_autotune.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem), _ip, _ip, _ip, _dp]
         L.hsrq_plan_workspace_bytes.restype = ctypes.c_size_t
         L.hsrq_plan_workspace_bytes.argtypes = [_vp]
         L.hsrq_plan_destroy.restype = None
@@ -201,6 +202,15 @@ class Context:
         _check(lib().hsrq_context_set_packing(self._h, shifts_per_wave), "set_packing")
         self.packing = shifts_per_wave
 
+    def set_tile(self, tile: int):
+        _check(lib().hsrq_context_set_tile(self._h, tile), "set_tile")
+
+    def reset_tuning(self):
+        """Back to the automatic tile, sweep layout and lane choice."""
+        self.set_tile(0)
+        self.set_packing(0)
+        self.set_lanes(0)
+
     def set_lanes(self, lanes: int):
         _check(lib().hsrq_context_set_lanes(self._h, lanes), "set_lanes")
 
@@ -249,6 +259,18 @@ class Context:
             out.append(st[pos:pos + w.numel()])
             pos += w.numel()
         return out
+
+    def autotune(self, items) -> dict:
+        """Times the problems for every tile / sweep layout / lane count and
+        keeps the fastest in this context (hsrq_autotune)."""
+        arr = self._problems(items)
+        t, pk, ln = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        _check(lib().hsrq_autotune(self._h, len(items), arr, ctypes.byref(t), ctypes.byref(pk),
+                                   ctypes.byref(ln), ctypes.byref(ms)), "hsrq_autotune")
+        self.packing = pk.value
+        return {"tile": t.value, "shifts_per_wave": pk.value, "lanes": ln.value,
+                "ms": round(ms.value, 3)}
 
     def plan(self, items, tlsz: int = 0) -> "Plan":
         return Plan(self, items, tlsz)

@@ -910,6 +910,75 @@ int hsrq_plan_lanes(const hsrq_plan* plan, int max_lanes, int* tile, int* packin
 
 void hsrq_plan_destroy(hsrq_plan* plan) { plan_free(plan); }
 
+int hsrq_autotune(hsrq_context* ctx, int count, const hsrq_problem* probs, int* tile,
+                  int* packing, int* lanes, double* ms) {
+  if (!ctx || count <= 0 || !probs) return HSRQ_EINVAL;
+  for (int i = 0; i < count; ++i)
+    if (validate_problem(probs[i])) return HSRQ_EINVAL;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const int t0 = ctx->tile, p0 = ctx->pack, l0 = ctx->lanes;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    last_hip_error() = he;
+    return HSRQ_EHIP;
+  }
+  double best = INFINITY;
+  int bt = 0, bp = 0, bl = 0, rc = HSRQ_OK;
+  static const int kTiles[] = {64, 128, 256};
+  static const int kPacks[] = {1, 4};
+  static const int kLanes[] = {1, 2, 3};
+  for (int t : kTiles)
+    for (int pk : kPacks)
+      for (int ln : kLanes) {
+        if (rc) break;
+        if (choose_pack(pk, t, 1 << 30) != pk) continue;   // layout unsupported at this tile
+        ctx->tile = t;
+        ctx->pack = pk;
+        ctx->lanes = ln;
+        hsrq_plan* p = nullptr;
+        rc = plan_build(ctx, count, probs, 0, -1, &p);
+        if (rc == HSRQ_ENOMEM) {   // this candidate does not fit: skip it
+          rc = HSRQ_OK;
+          continue;
+        }
+        // warm (captures the graph), then one timed replay
+        if (!rc) rc = hsrq_plan_execute_graph(p);
+        if (!rc && (he = hipEventRecord(e0, ctx->stream)) != hipSuccess) rc = HSRQ_EHIP;
+        if (!rc) rc = hsrq_plan_execute_graph(p);
+        if (!rc && (he = hipEventRecord(e1, ctx->stream)) != hipSuccess) rc = HSRQ_EHIP;
+        if (!rc && (he = hipEventSynchronize(e1)) != hipSuccess) rc = HSRQ_EHIP;
+        float el = 0.f;
+        if (!rc && (he = hipEventElapsedTime(&el, e0, e1)) != hipSuccess) rc = HSRQ_EHIP;
+        if (rc == HSRQ_EHIP) last_hip_error() = he;
+        plan_free(p);
+        if (!rc && el < best) {
+          best = el;
+          bt = t;
+          bp = pk;
+          bl = ln;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc || !(best < INFINITY)) {
+    ctx->tile = t0;
+    ctx->pack = p0;
+    ctx->lanes = l0;
+    return rc ? rc : HSRQ_ENOMEM;
+  }
+  ctx->tile = bt;
+  ctx->pack = bp;
+  ctx->lanes = bl;
+  if (tile) *tile = bt;
+  if (packing) *packing = bp;
+  if (lanes) *lanes = bl;
+  if (ms) *ms = best;
+  return HSRQ_OK;
+}
+
 int hsrq_residuals_device(hsrq_context* ctx, int n, const double* dH, int ldH, int m,
                           const double* dwr, const double* dvr, int ldvr, double* res) {
   if (!ctx || n <= 0 || m < 0 || ldH < n || ldvr < n || !dH || (m > 0 && (!dwr || !dvr || !res)))

@@ -126,6 +126,15 @@ int hsrq_plan_status(hsrq_plan *plan, int *status);
  * lanes or a negative code. */
 int hsrq_plan_lanes(const hsrq_plan *plan, int max_lanes, int *tile, int *packing,
                     int *problems);
+/* Autotuner: times the given problems as one plan (HIP-graph replay) for
+ * every tile size in {64, 128, 256}, diagonal-sweep layout in {1, 4} shifts
+ * per wavefront and lane count in {1, 2, 3}, and sets the fastest as the
+ * context's tile, packing and lanes (used by later plans and solves with
+ * tlsz <= 0).  The problems' vr are overwritten (no restarts run, wr keeps
+ * its order).  tile, packing, lanes, ms (host, may be NULL) receive the
+ * choice and its time. */
+int hsrq_autotune(hsrq_context *ctx, int count, const hsrq_problem *probs, int *tile,
+                  int *packing, int *lanes, double *ms);
 /* Bytes of device workspace held by the plan. */
 size_t hsrq_plan_workspace_bytes(const hsrq_plan *plan);
 void hsrq_plan_destroy(hsrq_plan *plan);

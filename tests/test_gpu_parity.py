@@ -140,6 +140,36 @@ def test_lanes_agree(ctx, lanes):
     assert np.array_equal(x1, x2) and (st & HSRQ_CONVERGED).all()
 
 
+def test_autotune_choice_is_consistent():
+    """The autotuner picks a supported configuration and a plan built after
+    it gives the same vectors as an explicit request of that configuration."""
+    from hsrq3in_amd import Context, colmajor_to_device
+    cases = [case(n, seed=300 + n, m=m) for n, m in [(200, 12), (450, 30), (700, 40)]]
+    def items():
+        return [(colmajor_to_device(H), torch.from_numpy(wr.copy()).cuda(),
+                 torch.zeros((len(wr), H.shape[0]), dtype=torch.float64, device="cuda"))
+                for H, wr in cases]
+    c = Context(0)
+    tuned = c.autotune(items())
+    assert tuned["tile"] in (64, 128, 256) and tuned["shifts_per_wave"] in (1, 4)
+    assert 1 <= tuned["lanes"] <= 3 and tuned["ms"] > 0
+    a = items()
+    p = c.plan(a)
+    p.execute(graph=True)
+    p.sync()
+    p.close()
+    c.close()
+    ref = Context(0, packing=tuned["shifts_per_wave"])
+    b = items()
+    p = ref.plan(b, tuned["tile"])
+    p.execute()
+    p.sync()
+    p.close()
+    ref.close()
+    for (_, _, va), (_, _, vb) in zip(a, b):
+        assert torch.equal(va, vb)
+
+
 def test_batch_equals_alone(ctx):
     from hsrq3in_amd import colmajor_to_device
     cases = [case(n, seed=100 + n, m=m) for n, m in [(90, 7), (257, 20), (64, 5), (500, 33)]]

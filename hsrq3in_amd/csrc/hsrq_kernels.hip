@@ -1055,7 +1055,7 @@ __global__ __launch_bounds__(256) void k_params(const ProbDev* __restrict__ prob
 // Block = 64 rows (one row block, inside one tile row I) x 64 columns of
 // [W | Z] interleaved (32 shifts).  Four waves, 32 x 32 each.
 // ---------------------------------------------------------------------------
-constexpr int KC = 32;          // K chunk staged in LDS
+constexpr int KC = 16;          // K chunk staged in LDS
 constexpr int GM = kRowBlock;   // 64 rows per block (one row block)
 constexpr int GN = kGemmCols;   // 64 columns per block (32 shifts)
 constexpr int LDA = GM + 4;
@@ -1078,7 +1078,9 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   // epilogue parameters of the block's shifts (loaded during the K loop)
   __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
   __shared__ int psy[GN / 2], psz[GN / 2];
-  __shared__ double red[2][GN];
+  // per-column maxima of the epilogue, in the H tile's LDS once the K loop
+  // is over (4 blocks of 256 threads fit a CU's LDS)
+  double(*red)[GN] = reinterpret_cast<double(*)[GN]>(&As[0][0]);
 
   const GemmItem it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
@@ -1166,10 +1168,8 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   const int g = lane >> 4;
   const bool isz = (g & 1) != 0;
   const int rl = lane & 15;
-  double rin[CT][4][2], yv[CT][4][2];
-  auto load_epilogue = [&]() {
-#pragma unroll
-    for (int c = 0; c < CT; ++c)
+  double rin[4][2], yv[4][2];   // one 16-column group at a time (registers)
+  auto load_epilogue = [&](int c) __attribute__((always_inline)) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
@@ -1180,8 +1180,8 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
           const int row = wm * 32 + a * 16 + rl;
           const int gi = r0 + row;
           const bool ok = okc && row < nr;
-          rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-          yv[c][r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+          rin[r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+          yv[r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
         }
       }
   };
@@ -1209,12 +1209,13 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     stash(buf ^ 1);
     __syncthreads();
   }
-  load_epilogue();   // the staging registers are free now
   mfma_chunk((nchunks - 1) & 1);
+  __syncthreads();   // As is reused for the column maxima below
 
-  // Epilogue.
+  // Epilogue, one 16-column group at a time.
 #pragma unroll
-  for (int c = 0; c < CT; ++c)
+  for (int c = 0; c < CT; ++c) {
+    load_epilogue(c);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
@@ -1232,12 +1233,12 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
           const int row = wm * 32 + a * 16 + rl;
           const int gi = r0 + row;
           if (row < nr) {
-            double val = fma(coef, rin[c][r][a], acc[a][c][r]);
+            double val = fma(coef, rin[r][a], acc[a][c][r]);
             if (gi == l - 1) val -= mu * b0;
             if (!isz) {
               Rout[gi + (int64_t)k * n] = val;
             } else {
-              const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
+              const double ynew = ldexp(yv[r][a], sy) - ldexp(val, sz);
               P.Y[gi + (int64_t)k * P.ldY] = ynew;
               cmax = fmax(cmax, fabs(ynew));
             }
@@ -1247,6 +1248,7 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
       const double x = row16_max(cmax);
       if (rl == 0) red[wm][cl] = x;
     }
+  }
   __syncthreads();
   if (tid < GN) {
     const int cl = tid;

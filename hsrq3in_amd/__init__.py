@@ -214,8 +214,10 @@ class Context:
     def set_lanes(self, lanes: int):
         _check(lib().hsrq_context_set_lanes(self._h, lanes), "set_lanes")
 
-    def set_lookahead(self, on: bool):
-        _check(lib().hsrq_context_set_lookahead(self._h, int(on)), "set_lookahead")
+    def set_lookahead(self, blocks_per_cu: int):
+        """0: off; k in 1..8: panel pass of the rows above on a second stream,
+        at most k workgroups per CU (plans of one lane); -1: automatic."""
+        _check(lib().hsrq_context_set_lookahead(self._h, int(blocks_per_cu)), "set_lookahead")
 
     def set_precision(self, prec: int):
         _check(lib().hsrq_context_set_precision(self._h, prec), "set_precision")

@@ -51,6 +51,7 @@ struct Lane {
   hipStream_t stream = nullptr;   // owned unless it is the context stream
   bool own_stream = false;
   hipStream_t side = nullptr;     // lookahead stream (context option)
+  int side_blocks = 0;            // grid cap of the panel passes on `side`
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
   std::vector<hipEvent_t> ev_p, ev_b;
 };
@@ -183,8 +184,20 @@ void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, in
   }
 }
 
-// Item tables of one lane, uploaded once.
-int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts) {
+// Workgroups per CU of the capped panel pass beside a diagonal sweep: the
+// room a sweep workgroup leaves (per SIMD 512 VGPRs, per CU 160 KiB LDS).
+// The looping panel pass takes 136 VGPRs per wave and 40 KiB of LDS per
+// block; k_diag with 1-2 rows per lane takes up to 224 VGPRs and 64 KiB,
+// the other sweep variants up to 256 VGPRs.  Measured, n = 8000, m = 800,
+// tile 128: 9.16 ms without lookahead, 8.83 / 7.60 / 8.87 / 8.58 ms with
+// 1 / 2 / 3 / 4 blocks per CU (MI355X).
+int auto_lookahead(int pack, int b) { return pack == 1 && b <= 128 ? 2 : 1; }
+
+// Item tables of one lane, uploaded once.  `single`: the plan has one lane,
+// which then may take a second stream for the panel pass of the rows above
+// (with several lanes the sweeps of one lane already overlap the panel
+// passes of another; the combination is not used).
+int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
   L->pack = choose_pack(ctx->pack, L->b, total_lane_shifts);
   std::vector<GemmItem> part;
   std::vector<RowItem> fin, init;
@@ -245,7 +258,11 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts) {
   if (cnt > 0)
     HIP_TRY(hipMemcpy(L->d_probs, L->probs.data(), cnt * sizeof(ProbDev), hipMemcpyHostToDevice));
   if (L->own_stream) HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-  if (ctx->lookahead) HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
+  const int la = !single ? 0 : ctx->lookahead < 0 ? auto_lookahead(L->pack, L->b) : ctx->lookahead;
+  if (la > 0) {
+    HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
+    L->side_blocks = la * std::max(ctx->cus, 1);
+  }
   HIP_TRY(hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L->ev_done, hipEventDisableTiming));
@@ -322,7 +339,7 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       L->probs.push_back(P);
       shifts += pc.m;
     }
-    rc = lane_build(ctx, L, shifts);
+    rc = lane_build(ctx, L, shifts, groups.size() == 1);
   }
   if (!rc) {
     e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
@@ -350,7 +367,9 @@ struct Prof {
 // the scaling exponents (after the previous step's panel pass of the rows
 // above), the panel pass of the next diagonal tile's rows, then the panel
 // pass of all rows above (on the lookahead stream when enabled, where it
-// overlaps the next diagonal sweep).
+// overlaps the next tile's panel pass and diagonal sweep: its grid is capped
+// so that every CU keeps room for a sweep workgroup, and the critical-path
+// launches run at raised wave priority).
 int lane_enqueue(Lane* L, Prof* prof) {
   hipStream_t st = L->stream;
   hipStream_t sd = L->side ? L->side : st;
@@ -390,14 +409,17 @@ int lane_enqueue(Lane* L, Prof* prof) {
     if (L->side && last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, L->ev_b[last_b], 0));
     rc = run(5, st, [&](hipStream_t x) { return launch_params(L->d_probs, L->pr[s], L->n_pr[s], x); });
     if (rc) return rc;
-    auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
-    rc = run(3, st, [&](hipStream_t x) { return upd(L->d_probs, L->ga[s], L->n_ga[s], x); });
-    if (rc) return rc;
-    if (L->side) {
+    if (L->side) {   // rows above: from here on, beside the next tile's rows and sweep
       HIP_TRY(hipEventRecord(L->ev_p[s], st));
       HIP_TRY(hipStreamWaitEvent(sd, L->ev_p[s], 0));
     }
-    rc = run(3, sd, [&](hipStream_t x) { return upd(L->d_probs, L->gb[s], L->n_gb[s], x); });
+    auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
+    const int hi = L->side ? 1 : 0;
+    rc = run(3, st, [&](hipStream_t x) { return upd(L->d_probs, L->ga[s], L->n_ga[s], x, 0, hi); });
+    if (rc) return rc;
+    rc = run(3, sd, [&](hipStream_t x) {
+      return upd(L->d_probs, L->gb[s], L->n_gb[s], x, L->side_blocks, 0);
+    });
     if (rc) return rc;
     if (L->side) HIP_TRY(hipEventRecord(L->ev_b[s], sd));
     last_b = s;
@@ -738,7 +760,8 @@ int hsrq_context_create(hsrq_context** ctx, int device) {
   hsrq_context* c = new (std::nothrow) hsrq_context();
   if (!c) return HSRQ_ENOMEM;
   c->device = dev;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     last_hip_error() = e;
     delete c;
@@ -780,9 +803,9 @@ int hsrq_context_set_lanes(hsrq_context* ctx, int lanes) {
   return HSRQ_OK;
 }
 
-int hsrq_context_set_lookahead(hsrq_context* ctx, int on) {
-  if (!ctx) return HSRQ_EINVAL;
-  ctx->lookahead = on ? 1 : 0;
+int hsrq_context_set_lookahead(hsrq_context* ctx, int blocks_per_cu) {
+  if (!ctx || blocks_per_cu < -1 || blocks_per_cu > 8) return HSRQ_EINVAL;
+  ctx->lookahead = blocks_per_cu;
   return HSRQ_OK;
 }
 

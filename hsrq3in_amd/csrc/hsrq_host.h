@@ -17,7 +17,9 @@ struct hsrq_context {
   int tile = 0;          // 0: choose per problem
   int exact = 0;
   int precision = HSRQ_PREC_FP64;
-  int lookahead = 0;     // 1: bulk panel pass on a second stream
+  int lookahead = -1;    // k > 0: bulk panel pass on a second stream, k blocks per
+                         // CU; 0: off; -1: automatic
+  int cus = 0;           // compute units of the device
   int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
   int lanes = 0;         // concurrent lanes of a plan (0: auto)
 };
@@ -81,8 +83,12 @@ hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, in
 hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
-hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t);
-hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t);
+// Panel pass; max_blocks > 0 caps the grid (blocks loop over the items), hi
+// raises the waves' priority (critical-path launches).
+hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t, int max_blocks = 0,
+                         int hi = 0);
+hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t,
+                           int max_blocks = 0, int hi = 0);
 hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);

@@ -178,6 +178,7 @@ __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
 template <int RPL>
 __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
                                               const K1Item* __restrict__ items) {
+  __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   constexpr int BMAX = 64 * RPL;
   constexpr int CW = (RPL == 4) ? 16 : 64;   // tile columns per LDS chunk
   __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
@@ -592,6 +593,7 @@ using IC = std::integral_constant<int, V>;
 template <int RPL>
 __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ probs,
                                                 const K1Item* __restrict__ items) {
+  __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   constexpr int BMAX = 16 * RPL;
   constexpr int CW = (BMAX <= 64) ? 64 : (BMAX <= 128 ? 32 : 16);   // LDS chunk columns
   __shared__ double Hs[CW * BMAX];
@@ -1021,6 +1023,7 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_params(const ProbDev* __restrict__ probs,
                                                 const K1Item* __restrict__ items) {
+  __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   const K1Item it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
   const int J = it.J;
@@ -1071,8 +1074,10 @@ constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 
 // Block = one 64-row block x 64 columns of [W | Z] (32 shifts); four waves,
 // 32 rows x 32 columns each (2 x 2 MFMA tiles).
+template <bool LOOP>
 __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
-                                                const GemmItem* __restrict__ items) {
+                                                const GemmItem* __restrict__ items, int count,
+                                                int hi) {
   __shared__ double As[2][KC * LDA64];   // As[buf][k * LDA64 + row]
   __shared__ double Bs[2][GN * LDB64];   // Bs[buf][col * LDB64 + k]
   // epilogue parameters of the block's shifts (loaded during the K loop)
@@ -1082,180 +1087,194 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   // is over (4 blocks of 256 threads fit a CU's LDS)
   double(*red)[GN] = reinterpret_cast<double(*)[GN]>(&As[0][0]);
 
-  const GemmItem it = items[blockIdx.x];
-  const ProbDev& P = probs[it.prob];
-  const int J = it.J;
-  const int m = P.m, n = P.n, b = P.b;
-  const int l = tile_first_row(P, J);
-  int tile, r0, nr;
-  rowblock_range(P, it.rb, tile, r0, nr);
-  const int col0 = it.cb * GN;
-  const int ncols = min(GN, 2 * m - col0);
+  // hi: the launch is on the critical path (raised wave priority over a
+  // concurrent bulk launch).  LOOP: grid-stride over the items, so that a
+  // bulk launch can be capped to leave room on every CU for the diagonal
+  // sweep; otherwise one block per item.
+  if (hi) __builtin_amdgcn_s_setprio(2);
+#pragma nounroll
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const GemmItem it = items[item];
+    const ProbDev& P = probs[it.prob];
+    const int J = it.J;
+    const int m = P.m, n = P.n, b = P.b;
+    const int l = tile_first_row(P, J);
+    int tile, r0, nr;
+    rowblock_range(P, it.rb, tile, r0, nr);
+    const int col0 = it.cb * GN;
+    const int ncols = min(GN, 2 * m - col0);
 
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int64_t ldH = P.ldH;
+    // the thread index is re-read per item (an empty asm makes it opaque), so
+    // that the index arithmetic derived from it is not held in registers
+    // across items
+    int tid = threadIdx.x;
+    if (LOOP) asm volatile("" : "+v"(tid));
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t ldH = P.ldH;
 
-  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;   // panel column 0 = l-1
-  const double* __restrict__ Bg = P.Bm[J & 1];
+    const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;   // panel column 0 = l-1
+    const double* __restrict__ Bg = P.Bm[J & 1];
 
-  // global -> register staging of one K chunk
-  constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
-  double pa[NPA], pb[NPB];
-  auto fetch = [&](int kc) {
-#pragma unroll
-    for (int i = 0; i < NPA; ++i) {
-      const int idx = tid + 256 * i;
-      const int kk = idx / GM, rr = idx - kk * GM;
-      pa[i] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH] : 0.0;
+    // global -> register staging of one K chunk
+    constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
+    double pa[NPA], pb[NPB];
+    auto fetch = [&](int kc) {
+  #pragma unroll
+      for (int i = 0; i < NPA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kk = idx / GM, rr = idx - kk * GM;
+        pa[i] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH] : 0.0;
+      }
+  #pragma unroll
+      for (int i = 0; i < NPB; ++i) {
+        const int idx = tid + 256 * i;
+        const int cc = idx / KC, k2 = idx - cc * KC;
+        pb[i] = (cc < ncols) ? Bg[(kc + k2) + (int64_t)(col0 + cc) * b] : 0.0;
+      }
+    };
+    auto stash = [&](int buf) {
+  #pragma unroll
+      for (int i = 0; i < NPA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kk = idx / GM, rr = idx - kk * GM;
+        As[buf][kk * LDA64 + rr] = pa[i];
+      }
+  #pragma unroll
+      for (int i = 0; i < NPB; ++i) {
+        const int idx = tid + 256 * i;
+        const int cc = idx / KC, k2 = idx - cc * KC;
+        Bs[buf][cc * LDB64 + k2] = pb[i];
+      }
+    };
+
+    // D^T = [W|Z]^T H_panel^T: MFMA operand A = B-matrix values, operand B = H
+    // values; the result D^T[col][row] sits in register col/4 of lane
+    // row + 16 (col % 4): one lane = one matrix row, four columns per tile.
+    double4_t acc[2][CT];
+  #pragma unroll
+    for (int a = 0; a < 2; ++a)
+  #pragma unroll
+      for (int c = 0; c < CT; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+    const int nchunks = b / KC;
+    fetch(0);
+    if (tid < GN / 2) {
+      const int k = col0 / 2 + tid;
+      if (k < m) {
+        const double* aux = P.aux[J & 1];
+        pmu[tid] = P.wr[k];
+        pP[tid] = aux[4 * k + 0];
+        pZl[tid] = aux[4 * k + 1];
+        pb0w[tid] = Bg[(int64_t)(2 * k) * b];
+        pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
+        psy[tid] = P.SY[(int64_t)tile * m + k];
+        psz[tid] = P.SZ[(int64_t)tile * m + k];
+      }
     }
-#pragma unroll
-    for (int i = 0; i < NPB; ++i) {
-      const int idx = tid + 256 * i;
-      const int cc = idx / KC, k2 = idx - cc * KC;
-      pb[i] = (cc < ncols) ? Bg[(kc + k2) + (int64_t)(col0 + cc) * b] : 0.0;
-    }
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < NPA; ++i) {
-      const int idx = tid + 256 * i;
-      const int kk = idx / GM, rr = idx - kk * GM;
-      As[buf][kk * LDA64 + rr] = pa[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NPB; ++i) {
-      const int idx = tid + 256 * i;
-      const int cc = idx / KC, k2 = idx - cc * KC;
-      Bs[buf][cc * LDB64 + k2] = pb[i];
-    }
-  };
+    stash(0);
+    __syncthreads();
 
-  // D^T = [W|Z]^T H_panel^T: MFMA operand A = B-matrix values, operand B = H
-  // values; the result D^T[col][row] sits in register col/4 of lane
-  // row + 16 (col % 4): one lane = one matrix row, four columns per tile.
-  double4_t acc[2][CT];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < CT; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
-
-  const int nchunks = b / KC;
-  fetch(0);
-  if (tid < GN / 2) {
-    const int k = col0 / 2 + tid;
-    if (k < m) {
-      const double* aux = P.aux[J & 1];
-      pmu[tid] = P.wr[k];
-      pP[tid] = aux[4 * k + 0];
-      pZl[tid] = aux[4 * k + 1];
-      pb0w[tid] = Bg[(int64_t)(2 * k) * b];
-      pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
-      psy[tid] = P.SY[(int64_t)tile * m + k];
-      psz[tid] = P.SZ[(int64_t)tile * m + k];
+    // Epilogue operands (R~ in, y), loaded while the last K chunk runs on
+    // MFMA: acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
+    // wn*GN/2 + c*16 + 4*r + lane/16; even lane groups hold W columns (R~
+    // out), odd ones Z columns (y).
+    const double* __restrict__ Rin = P.Rt[J & 1];
+    double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+    const int g = lane >> 4;
+    const bool isz = (g & 1) != 0;
+    const int rl = lane & 15;
+    double rin[4][2], yv[4][2];   // one 16-column group at a time (registers)
+    auto load_epilogue = [&](int c) __attribute__((always_inline)) {
+  #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
+          const int k = (col0 + cl) >> 1;
+          const bool okc = cl < ncols;
+  #pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int row = wm * 32 + a * 16 + rl;
+            const int gi = r0 + row;
+            const bool ok = okc && row < nr;
+            rin[r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+            yv[r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+          }
+        }
+    };
+    auto mfma_chunk = [&](int buf) __attribute__((always_inline)) {
+  #pragma unroll
+      for (int k4 = 0; k4 < KC; k4 += 4) {
+        const int kr = k4 + (lane >> 4);
+        double hv[2], zv[CT];
+  #pragma unroll
+        for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA64 + wm * 32 + a * 16 + (lane & 15)];
+  #pragma unroll
+        for (int c = 0; c < CT; ++c)
+          zv[c] = Bs[buf][(wn * (GN / 2) + c * 16 + (lane & 15)) * LDB64 + kr];
+  #pragma unroll
+        for (int a = 0; a < 2; ++a)
+  #pragma unroll
+          for (int c = 0; c < CT; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
+      }
+    };
+    for (int ch = 0; ch + 1 < nchunks; ++ch) {
+      const int buf = ch & 1;
+      fetch((ch + 1) * KC);
+      mfma_chunk(buf);
+      stash(buf ^ 1);
+      __syncthreads();
     }
-  }
-  stash(0);
-  __syncthreads();
+    mfma_chunk((nchunks - 1) & 1);
+    __syncthreads();   // As is reused for the column maxima below
 
-  // Epilogue operands (R~ in, y), loaded while the last K chunk runs on
-  // MFMA: acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
-  // wn*GN/2 + c*16 + 4*r + lane/16; even lane groups hold W columns (R~
-  // out), odd ones Z columns (y).
-  const double* __restrict__ Rin = P.Rt[J & 1];
-  double* __restrict__ Rout = P.Rt[(J + 1) & 1];
-  const int g = lane >> 4;
-  const bool isz = (g & 1) != 0;
-  const int rl = lane & 15;
-  double rin[4][2], yv[4][2];   // one 16-column group at a time (registers)
-  auto load_epilogue = [&](int c) __attribute__((always_inline)) {
-#pragma unroll
+    // Epilogue, one 16-column group at a time.
+  #pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      load_epilogue(c);
+  #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
+        const int ks = cl >> 1;
         const int k = (col0 + cl) >> 1;
         const bool okc = cl < ncols;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          const int row = wm * 32 + a * 16 + rl;
-          const int gi = r0 + row;
-          const bool ok = okc && row < nr;
-          rin[r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-          yv[r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
-        }
-      }
-  };
-  auto mfma_chunk = [&](int buf) __attribute__((always_inline)) {
-#pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      const int kr = k4 + (lane >> 4);
-      double hv[2], zv[CT];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA64 + wm * 32 + a * 16 + (lane & 15)];
-#pragma unroll
-      for (int c = 0; c < CT; ++c)
-        zv[c] = Bs[buf][(wn * (GN / 2) + c * 16 + (lane & 15)) * LDB64 + kr];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
-    }
-  };
-  for (int ch = 0; ch + 1 < nchunks; ++ch) {
-    const int buf = ch & 1;
-    fetch((ch + 1) * KC);
-    mfma_chunk(buf);
-    stash(buf ^ 1);
-    __syncthreads();
-  }
-  mfma_chunk((nchunks - 1) & 1);
-  __syncthreads();   // As is reused for the column maxima below
-
-  // Epilogue, one 16-column group at a time.
-#pragma unroll
-  for (int c = 0; c < CT; ++c) {
-    load_epilogue(c);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
-      const int ks = cl >> 1;
-      const int k = (col0 + cl) >> 1;
-      const bool okc = cl < ncols;
-      double cmax = 0.0;
-      if (okc) {
-        const double mu = pmu[ks];
-        const double coef = isz ? pZl[ks] : pP[ks];
-        const double b0 = isz ? pb0z[ks] : pb0w[ks];
-        const int sy = psy[ks], sz = psz[ks];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          const int row = wm * 32 + a * 16 + rl;
-          const int gi = r0 + row;
-          if (row < nr) {
-            double val = fma(coef, rin[r][a], acc[a][c][r]);
-            if (gi == l - 1) val -= mu * b0;
-            if (!isz) {
-              Rout[gi + (int64_t)k * n] = val;
-            } else {
-              const double ynew = ldexp(yv[r][a], sy) - ldexp(val, sz);
-              P.Y[gi + (int64_t)k * P.ldY] = ynew;
-              cmax = fmax(cmax, fabs(ynew));
+        double cmax = 0.0;
+        if (okc) {
+          const double mu = pmu[ks];
+          const double coef = isz ? pZl[ks] : pP[ks];
+          const double b0 = isz ? pb0z[ks] : pb0w[ks];
+          const int sy = psy[ks], sz = psz[ks];
+  #pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int row = wm * 32 + a * 16 + rl;
+            const int gi = r0 + row;
+            if (row < nr) {
+              double val = fma(coef, rin[r][a], acc[a][c][r]);
+              if (gi == l - 1) val -= mu * b0;
+              if (!isz) {
+                Rout[gi + (int64_t)k * n] = val;
+              } else {
+                const double ynew = ldexp(yv[r][a], sy) - ldexp(val, sz);
+                P.Y[gi + (int64_t)k * P.ldY] = ynew;
+                cmax = fmax(cmax, fabs(ynew));
+              }
             }
           }
         }
+        const double x = row16_max(cmax);
+        if (rl == 0) red[wm][cl] = x;
       }
-      const double x = row16_max(cmax);
-      if (rl == 0) red[wm][cl] = x;
     }
-  }
-  __syncthreads();
-  if (tid < GN) {
-    const int cl = tid;
-    if (cl < ncols && (cl & 1)) {
-      const int k = (col0 + cl) >> 1;
-      P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+    __syncthreads();
+    if (tid < GN) {
+      const int cl = tid;
+      if (cl < ncols && (cl & 1)) {
+        const int k = (col0 + cl) >> 1;
+        P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+      }
     }
+    if (!LOOP) break;
+    __syncthreads();   // LDS is reused by the block's next item
   }
 }
 
@@ -1269,193 +1288,209 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 // layout: D[i][j] in register i%4 of lane j + 16 (i/4); here D = D^T[col][row],
 // so a lane holds one row and four adjacent columns, W (even) and Z (odd).
 // ---------------------------------------------------------------------------
+template <bool LOOP>
 __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ probs,
-                                                  const GemmItem* __restrict__ items) {
+                                                  const GemmItem* __restrict__ items, int count,
+                                                int hi) {
   __shared__ float As[2][KC * LDA];
   __shared__ float Bs[2][KC * LDB];
   __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
   __shared__ int psy[GN / 2], psz[GN / 2], pez[GN / 2];
   __shared__ double red[2][GN];
 
-  const GemmItem it = items[blockIdx.x];
-  const ProbDev& P = probs[it.prob];
-  const int J = it.J;
-  const int m = P.m, n = P.n, b = P.b;
-  const int l = tile_first_row(P, J);
-  int tile, r0, nr;
-  rowblock_range(P, it.rb, tile, r0, nr);
-  const int col0 = it.cb * GN;
-  const int ncols = min(GN, 2 * m - col0);
-  const int eH = P.scal[0] > 0.0 ? ilogb(P.scal[0]) : 0;
+  // hi: the launch is on the critical path (raised wave priority over a
+  // concurrent bulk launch).  LOOP: grid-stride over the items, so that a
+  // bulk launch can be capped to leave room on every CU for the diagonal
+  // sweep; otherwise one block per item.
+  if (hi) __builtin_amdgcn_s_setprio(2);
+#pragma nounroll
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    const GemmItem it = items[item];
+    const ProbDev& P = probs[it.prob];
+    const int J = it.J;
+    const int m = P.m, n = P.n, b = P.b;
+    const int l = tile_first_row(P, J);
+    int tile, r0, nr;
+    rowblock_range(P, it.rb, tile, r0, nr);
+    const int col0 = it.cb * GN;
+    const int ncols = min(GN, 2 * m - col0);
+    const int eH = P.scal[0] > 0.0 ? ilogb(P.scal[0]) : 0;
 
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int64_t ldH = P.ldH;
+    // the thread index is re-read per item (an empty asm makes it opaque), so
+    // that the index arithmetic derived from it is not held in registers
+    // across items
+    int tid = threadIdx.x;
+    if (LOOP) asm volatile("" : "+v"(tid));
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t ldH = P.ldH;
 
-  const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;
-  const double* __restrict__ Bg = P.Bm[J & 1];
+    const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;
+    const double* __restrict__ Bg = P.Bm[J & 1];
 
-  constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
-  float pa[NPA], pb[NPB];
-  auto fetch = [&](int kc) {
-#pragma unroll
-    for (int i = 0; i < NPA; ++i) {
-      const int idx = tid + 256 * i;
-      const int kk = idx / GM, rr = idx - kk * GM;
-      pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH], -eH) : 0.0f;
-    }
-#pragma unroll
-    for (int i = 0; i < NPB; ++i) {
-      const int idx = tid + 256 * i;
-      const int cc = idx / KC, k2 = idx - cc * KC;
-      // Z columns (odd) scaled to max |Z| in [1, 2) by their own power of two
-      const int ez = (cc & 1) ? pez[cc >> 1] : 0;
-      pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b], -ez) : 0.0f;
-    }
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < NPA; ++i) {
-      const int idx = tid + 256 * i;
-      const int kk = idx / GM, rr = idx - kk * GM;
-      As[buf][kk * LDA + rr] = pa[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NPB; ++i) {
-      const int idx = tid + 256 * i;
-      const int cc = idx / KC, k2 = idx - cc * KC;
-      Bs[buf][k2 * LDB + cc] = pb[i];
-    }
-  };
-
-  float4_t acc[2][CT];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < CT; ++c) acc[a][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
-
-  const int nchunks = b / KC;
-  if (tid < GN / 2) {
-    const int k = col0 / 2 + tid;
-    int ez = 0;
-    if (k < m) {
-      const double* aux = P.aux[J & 1];
-      pmu[tid] = P.wr[k];
-      pP[tid] = aux[4 * k + 0];
-      pZl[tid] = aux[4 * k + 1];
-      pb0w[tid] = Bg[(int64_t)(2 * k) * b];
-      pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
-      psy[tid] = P.SY[(int64_t)tile * m + k];
-      psz[tid] = P.SZ[(int64_t)tile * m + k];
-      const double zn = aux[4 * k + 2];   // ||Z|| before the prescale 2^-p
-      if (zn > 0.0 && isfinite(zn)) ez = ilogb(zn) - (int)aux[4 * k + 3];
-    }
-    pez[tid] = ez;
-  }
-  __syncthreads();
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = ch & 1;
-    if (ch + 1 < nchunks) fetch((ch + 1) * KC);
-#pragma unroll
-    for (int k4 = 0; k4 < KC; k4 += 4) {
-      const int kr = k4 + (lane >> 4);
-      float hv[2], zv[CT];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
-#pragma unroll
-      for (int c = 0; c < CT; ++c)
-        zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-          acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(zv[c], hv[a], acc[a][c], 0, 0, 0);
-    }
-    if (ch + 1 < nchunks) stash(buf ^ 1);
-    __syncthreads();
-  }
-
-  // Epilogue.  acc[a][c][r]: row wm*32 + a*16 + lane%16,
-  //                          col wn*GN/2 + c*16 + 4*(lane/16) + r.
-  const double* __restrict__ Rin = P.Rt[J & 1];
-  double* __restrict__ Rout = P.Rt[(J + 1) & 1];
-  const int g = lane >> 4;
-  const int rl = lane & 15;
-  double rin[CT][4][2], yv[CT][4][2];
-#pragma unroll
-  for (int c = 0; c < CT; ++c)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
-      const int k = (col0 + cl) >> 1;
-      const bool okc = cl < ncols;
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const int row = wm * 32 + a * 16 + rl;
-        const int gi = r0 + row;
-        const bool ok = okc && row < nr;
-        rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-        yv[c][r][a] = (ok && (r & 1)) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+    constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
+    float pa[NPA], pb[NPB];
+    auto fetch = [&](int kc) {
+  #pragma unroll
+      for (int i = 0; i < NPA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kk = idx / GM, rr = idx - kk * GM;
+        pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH], -eH) : 0.0f;
       }
+  #pragma unroll
+      for (int i = 0; i < NPB; ++i) {
+        const int idx = tid + 256 * i;
+        const int cc = idx / KC, k2 = idx - cc * KC;
+        // Z columns (odd) scaled to max |Z| in [1, 2) by their own power of two
+        const int ez = (cc & 1) ? pez[cc >> 1] : 0;
+        pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b], -ez) : 0.0f;
+      }
+    };
+    auto stash = [&](int buf) {
+  #pragma unroll
+      for (int i = 0; i < NPA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kk = idx / GM, rr = idx - kk * GM;
+        As[buf][kk * LDA + rr] = pa[i];
+      }
+  #pragma unroll
+      for (int i = 0; i < NPB; ++i) {
+        const int idx = tid + 256 * i;
+        const int cc = idx / KC, k2 = idx - cc * KC;
+        Bs[buf][k2 * LDB + cc] = pb[i];
+      }
+    };
+
+    float4_t acc[2][CT];
+  #pragma unroll
+    for (int a = 0; a < 2; ++a)
+  #pragma unroll
+      for (int c = 0; c < CT; ++c) acc[a][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
+
+    const int nchunks = b / KC;
+    if (tid < GN / 2) {
+      const int k = col0 / 2 + tid;
+      int ez = 0;
+      if (k < m) {
+        const double* aux = P.aux[J & 1];
+        pmu[tid] = P.wr[k];
+        pP[tid] = aux[4 * k + 0];
+        pZl[tid] = aux[4 * k + 1];
+        pb0w[tid] = Bg[(int64_t)(2 * k) * b];
+        pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
+        psy[tid] = P.SY[(int64_t)tile * m + k];
+        psz[tid] = P.SZ[(int64_t)tile * m + k];
+        const double zn = aux[4 * k + 2];   // ||Z|| before the prescale 2^-p
+        if (zn > 0.0 && isfinite(zn)) ez = ilogb(zn) - (int)aux[4 * k + 3];
+      }
+      pez[tid] = ez;
     }
-  double cmaxs[CT][2];
-#pragma unroll
-  for (int c = 0; c < CT; ++c)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) cmaxs[c][h] = 0.0;
-#pragma unroll
-  for (int c = 0; c < CT; ++c)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const bool isz = (r & 1) != 0;
-      const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
-      const int ks = cl >> 1;
-      const int k = (col0 + cl) >> 1;
-      const bool okc = cl < ncols;
-      if (okc) {
-        const double mu = pmu[ks];
-        const double coef = isz ? pZl[ks] : pP[ks];
-        const double b0 = isz ? pb0z[ks] : pb0w[ks];
-        const int sy = psy[ks], sz = psz[ks];
-        const int es = eH + (isz ? pez[ks] : 0);
-#pragma unroll
+    __syncthreads();
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const int buf = ch & 1;
+      if (ch + 1 < nchunks) fetch((ch + 1) * KC);
+  #pragma unroll
+      for (int k4 = 0; k4 < KC; k4 += 4) {
+        const int kr = k4 + (lane >> 4);
+        float hv[2], zv[CT];
+  #pragma unroll
+        for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
+  #pragma unroll
+        for (int c = 0; c < CT; ++c)
+          zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
+  #pragma unroll
+        for (int a = 0; a < 2; ++a)
+  #pragma unroll
+          for (int c = 0; c < CT; ++c)
+            acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(zv[c], hv[a], acc[a][c], 0, 0, 0);
+      }
+      if (ch + 1 < nchunks) stash(buf ^ 1);
+      __syncthreads();
+    }
+
+    // Epilogue.  acc[a][c][r]: row wm*32 + a*16 + lane%16,
+    //                          col wn*GN/2 + c*16 + 4*(lane/16) + r.
+    const double* __restrict__ Rin = P.Rt[J & 1];
+    double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+    const int g = lane >> 4;
+    const int rl = lane & 15;
+    double rin[CT][4][2], yv[CT][4][2];
+  #pragma unroll
+    for (int c = 0; c < CT; ++c)
+  #pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
+        const int k = (col0 + cl) >> 1;
+        const bool okc = cl < ncols;
+  #pragma unroll
         for (int a = 0; a < 2; ++a) {
           const int row = wm * 32 + a * 16 + rl;
           const int gi = r0 + row;
-          if (row < nr) {
-            double val = fma(coef, rin[c][r][a], ldexp((double)acc[a][c][r], es));
-            if (gi == l - 1) val -= mu * b0;
-            if (!isz) {
-              Rout[gi + (int64_t)k * n] = val;
-            } else {
-              const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
-              P.Y[gi + (int64_t)k * P.ldY] = ynew;
-              cmaxs[c][r >> 1] = fmax(cmaxs[c][r >> 1], fabs(ynew));
+          const bool ok = okc && row < nr;
+          rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
+          yv[c][r][a] = (ok && (r & 1)) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+        }
+      }
+    double cmaxs[CT][2];
+  #pragma unroll
+    for (int c = 0; c < CT; ++c)
+  #pragma unroll
+      for (int h = 0; h < 2; ++h) cmaxs[c][h] = 0.0;
+  #pragma unroll
+    for (int c = 0; c < CT; ++c)
+  #pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool isz = (r & 1) != 0;
+        const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
+        const int ks = cl >> 1;
+        const int k = (col0 + cl) >> 1;
+        const bool okc = cl < ncols;
+        if (okc) {
+          const double mu = pmu[ks];
+          const double coef = isz ? pZl[ks] : pP[ks];
+          const double b0 = isz ? pb0z[ks] : pb0w[ks];
+          const int sy = psy[ks], sz = psz[ks];
+          const int es = eH + (isz ? pez[ks] : 0);
+  #pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int row = wm * 32 + a * 16 + rl;
+            const int gi = r0 + row;
+            if (row < nr) {
+              double val = fma(coef, rin[c][r][a], ldexp((double)acc[a][c][r], es));
+              if (gi == l - 1) val -= mu * b0;
+              if (!isz) {
+                Rout[gi + (int64_t)k * n] = val;
+              } else {
+                const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
+                P.Y[gi + (int64_t)k * P.ldY] = ynew;
+                cmaxs[c][r >> 1] = fmax(cmaxs[c][r >> 1], fabs(ynew));
+              }
             }
           }
         }
       }
+    // per-column maxima of |y| over the block's 64 rows (Z columns only)
+  #pragma unroll
+    for (int c = 0; c < CT; ++c)
+  #pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double x = row16_max(cmaxs[c][h]);
+        const int cl = wn * (GN / 2) + c * 16 + 4 * g + 2 * h + 1;
+        if (rl == 0) red[wm][cl] = x;
+      }
+    __syncthreads();
+    if (tid < GN) {
+      const int cl = tid;
+      if (cl < ncols && (cl & 1)) {
+        const int k = (col0 + cl) >> 1;
+        P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+      }
     }
-  // per-column maxima of |y| over the block's 64 rows (Z columns only)
-#pragma unroll
-  for (int c = 0; c < CT; ++c)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const double x = row16_max(cmaxs[c][h]);
-      const int cl = wn * (GN / 2) + c * 16 + 4 * g + 2 * h + 1;
-      if (rl == 0) red[wm][cl] = x;
-    }
-  __syncthreads();
-  if (tid < GN) {
-    const int cl = tid;
-    if (cl < ncols && (cl & 1)) {
-      const int k = (col0 + cl) >> 1;
-      P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
-    }
+    if (!LOOP) break;
+    __syncthreads();   // LDS is reused by the block's next item
   }
 }
 
@@ -1825,16 +1860,25 @@ hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count,
 }
 
 hipError_t launch_update(const ProbDev* d_probs, const GemmItem* items, int count,
-                         hipStream_t st) {
+                         hipStream_t st, int max_blocks, int hi) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update, dim3(count), dim3(256), 0, st, d_probs, items);
+  if (max_blocks > 0 && max_blocks < count)
+    hipLaunchKernelGGL(k_update<true>, dim3(max_blocks), dim3(256), 0, st, d_probs, items, count,
+                       hi);
+  else
+    hipLaunchKernelGGL(k_update<false>, dim3(count), dim3(256), 0, st, d_probs, items, count, hi);
   return hipGetLastError();
 }
 
 hipError_t launch_update32(const ProbDev* d_probs, const GemmItem* items, int count,
-                           hipStream_t st) {
+                           hipStream_t st, int max_blocks, int hi) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update32, dim3(count), dim3(256), 0, st, d_probs, items);
+  if (max_blocks > 0 && max_blocks < count)
+    hipLaunchKernelGGL(k_update32<true>, dim3(max_blocks), dim3(256), 0, st, d_probs, items,
+                       count, hi);
+  else
+    hipLaunchKernelGGL(k_update32<false>, dim3(count), dim3(256), 0, st, d_probs, items, count,
+                       hi);
   return hipGetLastError();
 }
 

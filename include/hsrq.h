@@ -62,9 +62,13 @@ int hsrq_context_set_packing(hsrq_context *ctx, int shifts_per_wave);
  * into one range per lane, with many problems whole problems are dealt to
  * lanes.  Results do not depend on the number of lanes. */
 int hsrq_context_set_lanes(hsrq_context *ctx, int lanes);
-/* 1: run the panel pass of the rows above the next diagonal tile on a
- * second stream, overlapping the next diagonal sweep (default 0). */
-int hsrq_context_set_lookahead(hsrq_context *ctx, int on);
+/* k in 1..8: run the panel pass of the rows above the next diagonal tile on
+ * a second stream, at most k workgroups per CU, overlapping the next tile's
+ * panel pass and diagonal sweep; 0: off; -1: automatic (default: on, with
+ * the most workgroups per CU that leave room for a sweep workgroup).
+ * Applies to plans of one lane; plans of several lanes ignore it.  Results
+ * do not depend on it. */
+int hsrq_context_set_lookahead(hsrq_context *ctx, int blocks_per_cu);
 /* Panel-pass precision: HSRQ_PREC_FP64 (default), HSRQ_PREC_FP32 or
  * HSRQ_PREC_FP32_RAW.  In the FP32 modes the panel GEMM runs in FP32 on MFMA
  * with FP32 accumulation; the diagonal sweeps, bounds, scale exponents,

@@ -6,6 +6,8 @@ separated from the eigenvalue by relative 1e-12, residual
 ||(H - wr_k I) x_k|| / (||H||_oo ||x_k|| 2^-52) <= 10 n, no Inf/NaN, the same
 convergence status and the same (possibly reordered) shift order.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -138,6 +140,29 @@ def test_lanes_agree(ctx, lanes):
     x2, _, st, _ = gpu_solve(c, H, wr, 128)
     c.close()
     assert np.array_equal(x1, x2) and (st & HSRQ_CONVERGED).all()
+
+
+@functools.lru_cache(maxsize=1)
+def _many_shift_case():
+    """n = 3000 and 600 of its real eigenvalues as shifts (computed once)."""
+    return case(3000, seed=91, m=600)
+
+
+@pytest.mark.parametrize("blocks_per_cu", [0, 1, 3])
+def test_lookahead_agrees(ctx, blocks_per_cu):
+    """The panel pass of the rows above on a second stream, capped to k
+    workgroups per CU (so that its blocks loop over the items: n = 3000,
+    m = 600 gives ~870 items per step, more than k * 256), gives the same
+    vectors bit for bit as the automatic setting (and as no lookahead)."""
+    from hsrq3in_amd import Context
+    H, wr = _many_shift_case()
+    x1, w1, st1, _ = gpu_solve(ctx, H, wr, 64)
+    c = Context(0, packing=ctx.packing)
+    c.set_lookahead(blocks_per_cu)
+    x2, w2, st2, res = gpu_solve(c, H, wr, 64)
+    c.close()
+    assert np.array_equal(x1, x2) and np.array_equal(w1, w2) and np.array_equal(st1, st2)
+    assert (st2 & HSRQ_CONVERGED).all() and res.max() <= 10 * 3000
 
 
 def test_autotune_choice_is_consistent():

@@ -41,13 +41,15 @@ PEAK_HBM_GBS = 8000.0
 
 
 def update_flops(n: int, m: int, b: int) -> float:
-    """Algorithmic FP64 flops of the panel passes: 2 (W and Z) x 2 l_J b m."""
+    """Algorithmic FP64 flops of the k_update launches: 2 (W and Z) x 2 r_J b m
+    for tile column J >= 2, r_J = the rows above tile J-1 (tile J-1's own rows
+    are updated inside its sweep, k_diag)."""
     T = (n + b - 1) // b
     b0 = n - (T - 1) * b
     tot = 0.0
-    for J in range(1, T):
-        l = b0 + (J - 1) * b
-        tot += 4.0 * l * b * m
+    for J in range(2, T):
+        r = b0 + (J - 2) * b
+        tot += 4.0 * r * b * m
     return tot
 
 

@@ -46,8 +46,8 @@ struct Lane {
   int steps = 0;
   std::vector<const K1Item*> k1, pr;
   std::vector<int> n_k1, n_pr;
-  std::vector<const GemmItem*> ga, gb;
-  std::vector<int> n_ga, n_gb;
+  std::vector<const GemmItem*> gb;   // panel pass of the rows above the next tile
+  std::vector<int> n_gb;
   hipStream_t stream = nullptr;   // owned unless it is the context stream
   bool own_stream = false;
   hipStream_t side = nullptr;     // lookahead stream (context option)
@@ -224,7 +224,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
                   align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem));
   for (const auto& x : st)
     nbytes += align_up(x.k1.size() * sizeof(K1Item)) + align_up(x.pr.size() * sizeof(K1Item)) +
-              align_up(x.ga.size() * sizeof(GemmItem)) + align_up(x.gb.size() * sizeof(GemmItem));
+              align_up(x.gb.size() * sizeof(GemmItem));
   nbytes = std::max(nbytes, kAlign);
   std::vector<char> host(nbytes);
   HIP_TRY(hipMalloc(&L->d_items, nbytes));
@@ -249,8 +249,6 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
     L->n_k1.push_back((int)x.k1.size());
     L->pr.push_back((const K1Item*)put(x.pr.data(), x.pr.size() * sizeof(K1Item)));
     L->n_pr.push_back((int)x.pr.size());
-    L->ga.push_back((const GemmItem*)put(x.ga.data(), x.ga.size() * sizeof(GemmItem)));
-    L->n_ga.push_back((int)x.ga.size());
     L->gb.push_back((const GemmItem*)put(x.gb.data(), x.gb.size() * sizeof(GemmItem)));
     L->n_gb.push_back((int)x.gb.size());
   }
@@ -363,13 +361,13 @@ struct Prof {
   std::vector<int> kind;
 };
 
-// Launch sequence of one lane on its stream.  Per step: the diagonal sweeps,
-// the scaling exponents (after the previous step's panel pass of the rows
-// above), the panel pass of the next diagonal tile's rows, then the panel
-// pass of all rows above (on the lookahead stream when enabled, where it
-// overlaps the next tile's panel pass and diagonal sweep: its grid is capped
-// so that every CU keeps room for a sweep workgroup, and the critical-path
-// launches run at raised wave priority).
+// Launch sequence of one lane on its stream.  Per step: the diagonal sweeps
+// (each starting with the panel pass of its own tile's rows), the scaling
+// exponents (after the previous step's panel pass of the rows above), then
+// the panel pass of the rows above the next diagonal tile (on the lookahead
+// stream when enabled, where it overlaps the next sweep: its grid is capped
+// so that every CU keeps room for a sweep workgroup, and the sweeps run at
+// raised wave priority).
 int lane_enqueue(Lane* L, Prof* prof) {
   hipStream_t st = L->stream;
   hipStream_t sd = L->side ? L->side : st;
@@ -405,22 +403,21 @@ int lane_enqueue(Lane* L, Prof* prof) {
       return launch_sweep(L->pack, L->b, L->d_probs, L->k1[s], L->n_k1[s], x);
     });
     if (rc) return rc;
-    if (L->n_ga[s] + L->n_gb[s] == 0) continue;
+    if (L->n_pr[s] == 0) continue;   // tile column 0: nothing above
     if (L->side && last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, L->ev_b[last_b], 0));
     rc = run(5, st, [&](hipStream_t x) { return launch_params(L->d_probs, L->pr[s], L->n_pr[s], x); });
     if (rc) return rc;
-    if (L->side) {   // rows above: from here on, beside the next tile's rows and sweep
+    if (L->side) {   // rows above: from here on, beside the next sweep
       HIP_TRY(hipEventRecord(L->ev_p[s], st));
       HIP_TRY(hipStreamWaitEvent(sd, L->ev_p[s], 0));
     }
     auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
-    const int hi = L->side ? 1 : 0;
-    rc = run(3, st, [&](hipStream_t x) { return upd(L->d_probs, L->ga[s], L->n_ga[s], x, 0, hi); });
-    if (rc) return rc;
-    rc = run(3, sd, [&](hipStream_t x) {
-      return upd(L->d_probs, L->gb[s], L->n_gb[s], x, L->side_blocks, 0);
-    });
-    if (rc) return rc;
+    if (L->n_gb[s] > 0) {
+      rc = run(3, sd, [&](hipStream_t x) {
+        return upd(L->d_probs, L->gb[s], L->n_gb[s], x, L->side_blocks);
+      });
+      if (rc) return rc;
+    }
     if (L->side) HIP_TRY(hipEventRecord(L->ev_b[s], sd));
     last_b = s;
   }

@@ -66,11 +66,11 @@ void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
 void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init);
 struct StepItems {
   std::vector<K1Item> k1, pr;      // diagonal sweep; scaling exponents
-  std::vector<GemmItem> ga, gb;    // panel pass: next diagonal tile; rows above
+  std::vector<GemmItem> gb;        // panel pass of the rows above the next diagonal
+                                   // tile (that tile's own rows: in its sweep)
   void clear() {
     k1.clear();
     pr.clear();
-    ga.clear();
     gb.clear();
   }
 };
@@ -83,12 +83,10 @@ hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, in
 hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
-// Panel pass; max_blocks > 0 caps the grid (blocks loop over the items), hi
-// raises the waves' priority (critical-path launches).
-hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t, int max_blocks = 0,
-                         int hi = 0);
+// Panel pass; max_blocks > 0 caps the grid (blocks loop over the items).
+hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t, int max_blocks = 0);
 hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t,
-                           int max_blocks = 0, int hi = 0);
+                           int max_blocks = 0);
 hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);

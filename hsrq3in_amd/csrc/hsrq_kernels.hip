@@ -155,6 +155,104 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
 }
 
 // ---------------------------------------------------------------------------
+// The panel pass of tile column J+1 over the rows of tile J, for one shift k:
+// the rows the sweep of tile J starts from.  It runs at the start of that
+// sweep (same launch, no separate critical-path GEMM launch):
+//   acc = H(tile J rows, l'-1 : l'+b-2) [W_k | Z_k],  l' = first row of J+1,
+//   R~ = acc_W + P_k R~in,  update = acc_Z + Zlast_k R~in  (minus mu B(0,.)
+//   on row l'-1), y = 2^sy y - 2^sz update      (update-task.c:103-248),
+// in FP64 FMAs in column order, the panel staged CW columns at a time through
+// the sweep's LDS chunk buffer (rows of the tile, column-major, ld BMAX).
+// Lanes hold rows r0 + STRIDE q.  Every sweep layout computes each row the
+// same way, so the layouts stay bit-identical.
+// ---------------------------------------------------------------------------
+template <int RPL, int STRIDE, int CW, int BMAX, int NL, int SPW>
+__device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0, int k, int sidx,
+                                                int l, int nb, int r0, double mu, double* Hs,
+                                                double* Bsh, double (&pf)[NL], double (&v)[RPL],
+                                                double (&y)[RPL]) {
+  const int b = P.b, m = P.m, n = P.n;
+  const int64_t ldH = P.ldH;
+  const double* __restrict__ Hq = P.H + (int64_t)(l + nb - 1) * ldH + l;
+  const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
+  // [W | Z] of the workgroup's SPW shifts, CW columns per chunk, in LDS as
+  // Bsh[c * 2 SPW + 2 s + {0: W, 1: Z}]
+  constexpr int NBT = CW * 2 * SPW, NB = (NBT + 255) / 256;
+  double pb[NB];
+  auto stage = [&](int chunk) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int c = idx / BMAX, r = idx - c * BMAX;
+      pf[i] = r < nb ? Hq[r + (int64_t)(chunk * CW + c) * ldH] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int j = idx / CW, c = idx - j * CW;
+      const int ks = min(k0 + (j >> 1), m - 1);
+      pb[i] = idx < NBT ? Bg[(int64_t)(2 * ks + (j & 1)) * b + chunk * CW + c] : 0.0;
+    }
+  };
+  double aw[RPL], az[RPL];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    aw[q] = 0.0;
+    az[q] = 0.0;
+  }
+  const int nch = b / CW;
+  stage(0);
+  for (int ch = 0; ch < nch; ++ch) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NL; ++i) Hs[threadIdx.x + 256 * i] = pf[i];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int j = idx / CW, c = idx - j * CW;
+      if (idx < NBT) Bsh[c * 2 * SPW + j] = pb[i];
+    }
+    __syncthreads();
+    if (ch + 1 < nch) stage(ch + 1);
+#pragma unroll 8
+    for (int cc = 0; cc < CW; ++cc) {
+      const double wv = Bsh[cc * 2 * SPW + 2 * sidx], zv = Bsh[cc * 2 * SPW + 2 * sidx + 1];
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const double h = Hs[cc * BMAX + r0 + STRIDE * q];
+        aw[q] = fma(h, wv, aw[q]);
+        az[q] = fma(h, zv, az[q]);
+      }
+    }
+  }
+  __syncthreads();   // the chunk buffer is the sweep's again
+  const double* aux = P.aux[(J + 1) & 1];
+  const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
+  const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
+  const int sy = P.SY[(int64_t)J * m + k], sz = P.SZ[(int64_t)J * m + k];
+  const double* __restrict__ Rin = P.Rt[(J + 1) & 1] + (int64_t)k * n + l;
+  const double* __restrict__ yg = P.Y + (int64_t)k * P.ldY + l;
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = r0 + STRIDE * q;
+    if (r < nb) {
+      const double rin = Rin[r];
+      double vw = fma(Pk, rin, aw[q]);
+      double vz = fma(Zl, rin, az[q]);
+      if (r == nb - 1) {
+        vw -= mu * b0w;
+        vz -= mu * b0z;
+      }
+      v[q] = vw;
+      y[q] = ldexp(yg[r], sy) - ldexp(vz, sz);
+    } else {
+      v[q] = 0.0;
+      y[q] = 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // K1: diagonal tile J, one wavefront per shift, RPL rows per lane.
 // ---------------------------------------------------------------------------
 // a[q] for a uniform slot q, as a binary tree of selects on the bits of q
@@ -182,6 +280,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   constexpr int BMAX = 64 * RPL;
   constexpr int CW = (RPL == 4) ? 16 : 64;   // tile columns per LDS chunk
   __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
+  __shared__ double Bsh[CW * 2 * 4];         // [W | Z] chunk of the 4 shifts
 
   const K1Item it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
@@ -198,12 +297,25 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
   double* yg = P.Y + (int64_t)k * P.ldY + l;
 
+  // H tile chunks: CW columns each, staged through registers one chunk ahead
+  // so the global loads overlap the sweep of the previous chunk.
+  constexpr int NL = CW * BMAX / 256;
+  double pf[NL];
+
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
+  if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
+    own_rows_update<RPL, 64, CW, BMAX, NL, 4>(P, J, it.k0, k, wave, l, nb, lane, mu, Hs, Bsh, pf,
+                                             v, y);
+  } else {             // the last tile column starts from the initial vectors
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      v[q] = r < nb ? Rin[r] : 0.0;
+      y[q] = r < nb ? yg[r] : 0.0;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
-    const int r = lane + 64 * q;
-    v[q] = r < nb ? Rin[r] : 0.0;
-    y[q] = r < nb ? yg[r] : 0.0;
     cr[q] = 1.0;
     sr[q] = 0.0;
   }
@@ -215,11 +327,6 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   int info = 0;
   const double slack = 1.0 + 0x1p-40;
   const bool exact = P.force_exact != 0;
-
-  // H tile chunks: CW columns each, staged through registers one chunk ahead
-  // so the global loads overlap the sweep of the previous chunk.
-  constexpr int NL = CW * BMAX / 256;
-  double pf[NL];
   auto issue = [&](int chunk) {
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
@@ -597,6 +704,7 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   constexpr int BMAX = 16 * RPL;
   constexpr int CW = (BMAX <= 64) ? 64 : (BMAX <= 128 ? 32 : 16);   // LDS chunk columns
   __shared__ double Hs[CW * BMAX];
+  __shared__ double Bsh[CW * 2 * 16];   // [W | Z] chunk of the 16 shifts
 
   const K1Item it = items[blockIdx.x];
   const ProbDev& P = probs[it.prob];
@@ -614,12 +722,23 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
   double* yg = P.Y + (int64_t)k * P.ldY + l;
 
+  constexpr int NL = CW * BMAX / 256;
+  double pf[NL];
+
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
+  if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
+    own_rows_update<RPL, 16, CW, BMAX, NL, 16>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
+                                               Bsh, pf, v, y);
+  } else {             // the last tile column starts from the initial vectors
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = t + 16 * q;
+      v[q] = r < nb ? Rin[r] : 0.0;
+      y[q] = r < nb ? yg[r] : 0.0;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
-    const int r = t + 16 * q;
-    v[q] = r < nb ? Rin[r] : 0.0;
-    y[q] = r < nb ? yg[r] : 0.0;
     cr[q] = 1.0;
     sr[q] = 0.0;
   }
@@ -631,9 +750,6 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   int info = 0;
   const double slack = 1.0 + 0x1p-40;
   const bool exact = P.force_exact != 0;
-
-  constexpr int NL = CW * BMAX / 256;
-  double pf[NL];
   auto issue = [&](int chunk) __attribute__((always_inline)) {
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
@@ -1076,8 +1192,7 @@ constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 // 32 rows x 32 columns each (2 x 2 MFMA tiles).
 template <bool LOOP>
 __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
-                                                const GemmItem* __restrict__ items, int count,
-                                                int hi) {
+                                                const GemmItem* __restrict__ items, int count) {
   __shared__ double As[2][KC * LDA64];   // As[buf][k * LDA64 + row]
   __shared__ double Bs[2][GN * LDB64];   // Bs[buf][col * LDB64 + k]
   // epilogue parameters of the block's shifts (loaded during the K loop)
@@ -1087,11 +1202,9 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
   // is over (4 blocks of 256 threads fit a CU's LDS)
   double(*red)[GN] = reinterpret_cast<double(*)[GN]>(&As[0][0]);
 
-  // hi: the launch is on the critical path (raised wave priority over a
-  // concurrent bulk launch).  LOOP: grid-stride over the items, so that a
-  // bulk launch can be capped to leave room on every CU for the diagonal
-  // sweep; otherwise one block per item.
-  if (hi) __builtin_amdgcn_s_setprio(2);
+  // LOOP: grid-stride over the items, so that a launch beside a diagonal
+  // sweep can be capped to leave room for it on every CU; otherwise one
+  // block per item.
 #pragma nounroll
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const GemmItem it = items[item];
@@ -1290,19 +1403,16 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 // ---------------------------------------------------------------------------
 template <bool LOOP>
 __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ probs,
-                                                  const GemmItem* __restrict__ items, int count,
-                                                int hi) {
+                                                  const GemmItem* __restrict__ items, int count) {
   __shared__ float As[2][KC * LDA];
   __shared__ float Bs[2][KC * LDB];
   __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
   __shared__ int psy[GN / 2], psz[GN / 2], pez[GN / 2];
   __shared__ double red[2][GN];
 
-  // hi: the launch is on the critical path (raised wave priority over a
-  // concurrent bulk launch).  LOOP: grid-stride over the items, so that a
-  // bulk launch can be capped to leave room on every CU for the diagonal
-  // sweep; otherwise one block per item.
-  if (hi) __builtin_amdgcn_s_setprio(2);
+  // LOOP: grid-stride over the items, so that a launch beside a diagonal
+  // sweep can be capped to leave room for it on every CU; otherwise one
+  // block per item.
 #pragma nounroll
   for (int item = blockIdx.x; item < count; item += gridDim.x) {
     const GemmItem it = items[item];
@@ -1860,25 +1970,23 @@ hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count,
 }
 
 hipError_t launch_update(const ProbDev* d_probs, const GemmItem* items, int count,
-                         hipStream_t st, int max_blocks, int hi) {
+                         hipStream_t st, int max_blocks) {
   if (count <= 0) return hipSuccess;
   if (max_blocks > 0 && max_blocks < count)
-    hipLaunchKernelGGL(k_update<true>, dim3(max_blocks), dim3(256), 0, st, d_probs, items, count,
-                       hi);
+    hipLaunchKernelGGL(k_update<true>, dim3(max_blocks), dim3(256), 0, st, d_probs, items, count);
   else
-    hipLaunchKernelGGL(k_update<false>, dim3(count), dim3(256), 0, st, d_probs, items, count, hi);
+    hipLaunchKernelGGL(k_update<false>, dim3(count), dim3(256), 0, st, d_probs, items, count);
   return hipGetLastError();
 }
 
 hipError_t launch_update32(const ProbDev* d_probs, const GemmItem* items, int count,
-                           hipStream_t st, int max_blocks, int hi) {
+                           hipStream_t st, int max_blocks) {
   if (count <= 0) return hipSuccess;
   if (max_blocks > 0 && max_blocks < count)
     hipLaunchKernelGGL(k_update32<true>, dim3(max_blocks), dim3(256), 0, st, d_probs, items,
-                       count, hi);
+                       count);
   else
-    hipLaunchKernelGGL(k_update32<false>, dim3(count), dim3(256), 0, st, d_probs, items, count,
-                       hi);
+    hipLaunchKernelGGL(k_update32<false>, dim3(count), dim3(256), 0, st, d_probs, items, count);
   return hipGetLastError();
 }
 

@@ -139,24 +139,21 @@ void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out) {
   for (int k0 = 0; k0 < P.m; k0 += k1step) out.k1.push_back({i, J, k0, 0});
   if (J == 0) return;
   for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) out.pr.push_back({i, J, k0, 0});
-  // Panel pass in row-block pairs, XCD-aware order: blocks are dealt to the
-  // 8 XCDs round robin, so pairs 8t .. 8t+7 go out interleaved and every XCD
-  // sweeps all column blocks of one pair from its own L2.
-  const int nrbJ = tile_first_rowblock(P, J);
+  // Panel pass of the rows above tile J-1 (tile J-1's own rows are updated
+  // at the start of its sweep), in row-block pairs, XCD-aware order: blocks
+  // are dealt to the 8 XCDs round robin, so pairs 8t .. 8t+7 go out
+  // interleaved and every XCD sweeps all column blocks of one pair from its
+  // own L2.
   const int rbn = tile_first_rowblock(P, J - 1);
   const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
-  auto emit = [&](std::vector<GemmItem>& dst, int rs, int re) {
-    std::vector<std::pair<int, int>> pairs;
-    for (int rb = rs; rb < re; rb += kGemmRowBlocks)
-      pairs.push_back({rb, std::min(kGemmRowBlocks, re - rb)});
-    for (size_t t = 0; t < pairs.size(); t += 8) {
-      const size_t te = std::min(pairs.size(), t + 8);
-      for (int cb = 0; cb < ncb; ++cb)
-        for (size_t q = t; q < te; ++q) dst.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
-    }
-  };
-  emit(out.ga, rbn, nrbJ);
-  emit(out.gb, 0, rbn);
+  std::vector<std::pair<int, int>> pairs;
+  for (int rb = 0; rb < rbn; rb += kGemmRowBlocks)
+    pairs.push_back({rb, std::min(kGemmRowBlocks, rbn - rb)});
+  for (size_t t = 0; t < pairs.size(); t += 8) {
+    const size_t te = std::min(pairs.size(), t + 8);
+    for (int cb = 0; cb < ncb; ++cb)
+      for (size_t q = t; q < te; ++q) out.gb.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
+  }
 }
 
 void add_bt_items(const ProbDev& P, int i, std::vector<ShiftItem>& bt) {

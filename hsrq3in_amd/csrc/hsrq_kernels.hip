@@ -172,9 +172,23 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
                                                 double* Bsh, double (&pf)[NL], double (&v)[RPL],
                                                 double (&y)[RPL]) {
   const int b = P.b, m = P.m, n = P.n;
+  // epilogue operands first: their loads overlap the panel product
+  const double* aux = P.aux[(J + 1) & 1];
+  const double* __restrict__ Rin = P.Rt[(J + 1) & 1] + (int64_t)k * n + l;
+  const double* __restrict__ yg = P.Y + (int64_t)k * P.ldY + l;
+  double rin[RPL], yo[RPL];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int r = r0 + STRIDE * q;
+    rin[q] = r < nb ? Rin[r] : 0.0;
+    yo[q] = r < nb ? yg[r] : 0.0;
+  }
+  const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
+  const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
+  const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
+  const int sy = P.SY[(int64_t)J * m + k], sz = P.SZ[(int64_t)J * m + k];
   const int64_t ldH = P.ldH;
   const double* __restrict__ Hq = P.H + (int64_t)(l + nb - 1) * ldH + l;
-  const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
   // [W | Z] of the workgroup's SPW shifts, CW columns per chunk, in LDS as
   // Bsh[c * 2 SPW + 2 s + {0: W, 1: Z}]
   constexpr int NBT = CW * 2 * SPW, NB = (NBT + 255) / 256;
@@ -226,25 +240,18 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
     }
   }
   __syncthreads();   // the chunk buffer is the sweep's again
-  const double* aux = P.aux[(J + 1) & 1];
-  const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
-  const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
-  const int sy = P.SY[(int64_t)J * m + k], sz = P.SZ[(int64_t)J * m + k];
-  const double* __restrict__ Rin = P.Rt[(J + 1) & 1] + (int64_t)k * n + l;
-  const double* __restrict__ yg = P.Y + (int64_t)k * P.ldY + l;
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int r = r0 + STRIDE * q;
     if (r < nb) {
-      const double rin = Rin[r];
-      double vw = fma(Pk, rin, aw[q]);
-      double vz = fma(Zl, rin, az[q]);
+      double vw = fma(Pk, rin[q], aw[q]);
+      double vz = fma(Zl, rin[q], az[q]);
       if (r == nb - 1) {
         vw -= mu * b0w;
         vz -= mu * b0z;
       }
       v[q] = vw;
-      y[q] = ldexp(yg[r], sy) - ldexp(vz, sz);
+      y[q] = ldexp(yo[q], sy) - ldexp(vz, sz);
     } else {
       v[q] = 0.0;
       y[q] = 0.0;
@@ -302,11 +309,22 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   constexpr int NL = CW * BMAX / 256;
   double pf[NL];
 
+  auto issue = [&](int chunk) {
+    const int c0 = chunk * CW;
+    const int nc = min(CW, nb - c0);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int c = idx / BMAX, r = idx - c * BMAX;
+      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+    }
+  };
+
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
     own_rows_update<RPL, 64, CW, BMAX, NL, 4>(P, J, it.k0, k, wave, l, nb, lane, mu, Hs, Bsh, pf,
                                              v, y);
-  } else {             // the last tile column starts from the initial vectors
+  } else {       // the last tile column starts from the initial vectors
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = lane + 64 * q;
@@ -327,16 +345,6 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   int info = 0;
   const double slack = 1.0 + 0x1p-40;
   const bool exact = P.force_exact != 0;
-  auto issue = [&](int chunk) {
-    const int c0 = chunk * CW;
-    const int nc = min(CW, nb - c0);
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int c = idx / BMAX, r = idx - c * BMAX;
-      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
-    }
-  };
   int cur = -1;
   auto column = [&](int col) -> const double* {
     const int ch = col / CW;
@@ -725,11 +733,22 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   constexpr int NL = CW * BMAX / 256;
   double pf[NL];
 
+  auto issue = [&](int chunk) __attribute__((always_inline)) {
+    const int c0 = chunk * CW;
+    const int nc = min(CW, nb - c0);
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int c = idx / BMAX, r = idx - c * BMAX;
+      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+    }
+  };
+
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
     own_rows_update<RPL, 16, CW, BMAX, NL, 16>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
                                                Bsh, pf, v, y);
-  } else {             // the last tile column starts from the initial vectors
+  } else {       // the last tile column starts from the initial vectors
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = t + 16 * q;
@@ -750,16 +769,6 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   int info = 0;
   const double slack = 1.0 + 0x1p-40;
   const bool exact = P.force_exact != 0;
-  auto issue = [&](int chunk) __attribute__((always_inline)) {
-    const int c0 = chunk * CW;
-    const int nc = min(CW, nb - c0);
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int c = idx / BMAX, r = idx - c * BMAX;
-      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
-    }
-  };
   int cur = -1;
   auto column = [&](int col) __attribute__((always_inline)) -> const double* {
     const int ch = col / CW;

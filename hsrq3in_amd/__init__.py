@@ -303,7 +303,7 @@ class Plan:
         f = lib().hsrq_plan_execute_graph if graph else lib().hsrq_plan_execute
         _check(f(self._h), "hsrq_plan_execute")
 
-    KERNELS = ("norms", "init", "diag", "update", "backtransform", "params")
+    KERNELS = ("norms", "init", "diag", "update", "backtransform")
 
     def execute_profiled(self) -> dict:
         """Synchronous run with HIP events per launch: {kernel: (ms, launches)}."""

@@ -44,8 +44,8 @@ struct Lane {
   const ShiftItem* bt = nullptr;
   int n_bt = 0;
   int steps = 0;
-  std::vector<const K1Item*> k1, pr;
-  std::vector<int> n_k1, n_pr;
+  std::vector<const K1Item*> k1;
+  std::vector<int> n_k1;
   std::vector<const GemmItem*> gb;   // panel pass of the rows above the next tile
   std::vector<int> n_gb;
   hipStream_t stream = nullptr;   // owned unless it is the context stream
@@ -223,7 +223,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
   size_t nbytes = align_up(part.size() * sizeof(GemmItem)) + align_up(fin.size() * sizeof(RowItem)) +
                   align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem));
   for (const auto& x : st)
-    nbytes += align_up(x.k1.size() * sizeof(K1Item)) + align_up(x.pr.size() * sizeof(K1Item)) +
+    nbytes += align_up(x.k1.size() * sizeof(K1Item)) +
               align_up(x.gb.size() * sizeof(GemmItem));
   nbytes = std::max(nbytes, kAlign);
   std::vector<char> host(nbytes);
@@ -247,8 +247,6 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
   for (const auto& x : st) {
     L->k1.push_back((const K1Item*)put(x.k1.data(), x.k1.size() * sizeof(K1Item)));
     L->n_k1.push_back((int)x.k1.size());
-    L->pr.push_back((const K1Item*)put(x.pr.data(), x.pr.size() * sizeof(K1Item)));
-    L->n_pr.push_back((int)x.pr.size());
     L->gb.push_back((const GemmItem*)put(x.gb.data(), x.gb.size() * sizeof(GemmItem)));
     L->n_gb.push_back((int)x.gb.size());
   }
@@ -355,19 +353,18 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
 }
 
 // Per-launch timing (kinds: 0 norms, 1 init, 2 diag, 3 update,
-// 4 back-transform, 5 params).
+// 4 back-transform; 5 reserved).
 struct Prof {
   std::vector<hipEvent_t> ev;
   std::vector<int> kind;
 };
 
 // Launch sequence of one lane on its stream.  Per step: the diagonal sweeps
-// (each starting with the panel pass of its own tile's rows), the scaling
-// exponents (after the previous step's panel pass of the rows above), then
-// the panel pass of the rows above the next diagonal tile (on the lookahead
-// stream when enabled, where it overlaps the next sweep: its grid is capped
-// so that every CU keeps room for a sweep workgroup, and the sweeps run at
-// raised wave priority).
+// (each starting with the panel pass of its own tile's rows), then the panel
+// pass of the rows above the next diagonal tile (on the lookahead stream when
+// enabled, where it overlaps the next sweep: its grid is capped so that every
+// CU keeps room for a sweep workgroup, and the sweeps run at raised wave
+// priority).  Both kernels derive their scale exponents themselves.
 int lane_enqueue(Lane* L, Prof* prof) {
   hipStream_t st = L->stream;
   hipStream_t sd = L->side ? L->side : st;
@@ -397,29 +394,27 @@ int lane_enqueue(Lane* L, Prof* prof) {
     HIP_TRY(hipEventRecord(L->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(sd, L->ev_fork, 0));
   }
-  int last_b = -1;
+  auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
   for (int s = 0; s < L->steps; ++s) {
+    // The sweep of step s starts from its tile's rows as the panel pass of
+    // step s-2 left them, and overwrites the coefficient buffers that pass
+    // read; the panel pass of step s-1 (rows above the tile) may still run.
+    if (L->side && s >= 2 && L->n_gb[s - 2] > 0)
+      HIP_TRY(hipStreamWaitEvent(st, L->ev_b[s - 2], 0));
     rc = run(2, st, [&](hipStream_t x) {
       return launch_sweep(L->pack, L->b, L->d_probs, L->k1[s], L->n_k1[s], x);
     });
     if (rc) return rc;
-    if (L->n_pr[s] == 0) continue;   // tile column 0: nothing above
-    if (L->side && last_b >= 0) HIP_TRY(hipStreamWaitEvent(st, L->ev_b[last_b], 0));
-    rc = run(5, st, [&](hipStream_t x) { return launch_params(L->d_probs, L->pr[s], L->n_pr[s], x); });
-    if (rc) return rc;
-    if (L->side) {   // rows above: from here on, beside the next sweep
+    if (L->n_gb[s] == 0) continue;
+    if (L->side) {   // rows above: beside the next sweep
       HIP_TRY(hipEventRecord(L->ev_p[s], st));
       HIP_TRY(hipStreamWaitEvent(sd, L->ev_p[s], 0));
     }
-    auto upd = L->prec == HSRQ_PREC_FP64 ? launch_update : launch_update32;
-    if (L->n_gb[s] > 0) {
-      rc = run(3, sd, [&](hipStream_t x) {
-        return upd(L->d_probs, L->gb[s], L->n_gb[s], x, L->side_blocks);
-      });
-      if (rc) return rc;
-    }
+    rc = run(3, sd, [&](hipStream_t x) {
+      return upd(L->d_probs, L->gb[s], L->n_gb[s], x, L->side_blocks);
+    });
+    if (rc) return rc;
     if (L->side) HIP_TRY(hipEventRecord(L->ev_b[s], sd));
-    last_b = s;
   }
   if (L->side) {   // join (also without work: graph capture needs it)
     HIP_TRY(hipEventRecord(L->ev_join, sd));

@@ -65,12 +65,11 @@ void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
                     std::vector<RowItem>& fin);
 void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init);
 struct StepItems {
-  std::vector<K1Item> k1, pr;      // diagonal sweep; scaling exponents
+  std::vector<K1Item> k1;          // diagonal sweep
   std::vector<GemmItem> gb;        // panel pass of the rows above the next diagonal
                                    // tile (that tile's own rows: in its sweep)
   void clear() {
     k1.clear();
-    pr.clear();
     gb.clear();
   }
 };
@@ -87,7 +86,6 @@ hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_
 hipError_t launch_update(const ProbDev*, const GemmItem*, int, hipStream_t, int max_blocks = 0);
 hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t,
                            int max_blocks = 0);
-hipError_t launch_params(const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);
 hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,

@@ -33,10 +33,11 @@ struct ProbDev {
   double* Bm[2];      // GEMM right operand per tile-column parity, b x 2m (ld b):
                       // col 2k = W_k, 2k+1 = Z_k
   double* aux[2];     // per parity, 4 per shift: P_k, Zlast_k (prescaled), ||Z_k||, prescale
-  int* E;             // scale exponents, T x m  (E[t*m + k])
-  int* SY;            // epilogue exponents for y,  T x m
-  int* SZ;            // epilogue exponents for the update, T x m
-  double* Ynb;        // per row block max |y|, nrb x m
+  int* Ep[2];         // scale exponent of tile t after the panel pass of tile
+                      // column s, in Ep[s & 1][t*m + k]; Ep[T & 1]: initial
+  int* Ef;            // final scale exponent of tile t (after its sweep), T x m
+  double* Ynb[2];     // per row block max |y| after the panel pass of tile
+                      // column s, in Ynb[s & 1] (nrb x m); Ynb[T & 1]: initial
   double* rowinf;     // ||H(I,:)||_oo per tile row, T
   double* diagn;      // ||H(J,J)||_oo per diagonal tile, T
   double* part;       // partial absolute row sums, nrb x n

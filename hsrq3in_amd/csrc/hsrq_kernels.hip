@@ -147,11 +147,43 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
     P.Rt[(P.T - 1) & 1][row + (int64_t)k * n] = h;
   }
   const double mx = wave_max(fabs(yv));
-  if (lane == 0) P.Ynb[(int64_t)it.rb * P.m + k] = mx;
+  if (lane == 0) P.Ynb[P.T & 1][(int64_t)it.rb * P.m + k] = mx;
   if (it.rb == 0) {
-    for (int t = lane; t < P.T; t += 64) P.E[(int64_t)t * P.m + k] = 0;
+    for (int t = lane; t < P.T; t += 64) P.Ep[P.T & 1][(int64_t)t * P.m + k] = 0;
     if (lane == 0) P.status[k] = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Scale exponents of the panel pass of tile column J for tile row I < J and
+// shift k: consolidate the scales of y_I and Z_J (update-task.c:68-88), then
+// one protection for the whole fused update (update-task.c:103-248):
+// |update(i)| <= (||H(I,:)||_oo + |mu|) ||Z|| + |R~(i)| |Zlast| and
+// |R~(i)| <= ||(H - mu I)(i,:)||_2, so t_I = 2 (||H(I,:)||_oo + |mu|).
+// Reads the state the panel pass of tile column J+1 left (Ep, Ynb of parity
+// J+1) and tile J's final exponent; returns tile I's new exponent (for
+// parity J) and the epilogue exponents sy (of y) and sz (of the update).
+// Every block that updates rows of tile I derives the same values.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int update_exponents(const ProbDev& P, int I, int J, int k, double mu,
+                                                int& sy, int& sz) {
+  const int m = P.m;
+  const double* __restrict__ Yb = P.Ynb[(J + 1) & 1];
+  const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
+  double yn = 0.0;
+  for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, Yb[(int64_t)rb * m + k]);
+  const int EI = P.Ep[(J + 1) & 1][(int64_t)I * m + k];
+  const int EJ = P.Ef[(int64_t)J * m + k];
+  const double zn = P.aux[J & 1][4 * k + 2];
+  const int p = (int)P.aux[J & 1][4 * k + 3];
+  const int c0 = min(EI, EJ);
+  const double yI = ldexp(yn, c0 - EI);
+  const double zJ = ldexp(zn, c0 - EJ);
+  const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
+  const int ec = c0 + pow2_exp(protect_update(tI, zJ, yI));
+  sy = ec - EI;
+  sz = ec - EJ + p;
+  return ec;
 }
 
 // ---------------------------------------------------------------------------
@@ -170,7 +202,7 @@ template <int RPL, int STRIDE, int CW, int BMAX, int NL, int SPW>
 __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0, int k, int sidx,
                                                 int l, int nb, int r0, double mu, double* Hs,
                                                 double* Bsh, double (&pf)[NL], double (&v)[RPL],
-                                                double (&y)[RPL]) {
+                                                double (&y)[RPL], int& E0) {
   const int b = P.b, m = P.m, n = P.n;
   // epilogue operands first: their loads overlap the panel product
   const double* aux = P.aux[(J + 1) & 1];
@@ -186,7 +218,8 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
   const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
   const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
   const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
-  const int sy = P.SY[(int64_t)J * m + k], sz = P.SZ[(int64_t)J * m + k];
+  int sy, sz;
+  E0 = update_exponents(P, J, J + 1, k, mu, sy, sz);
   const int64_t ldH = P.ldH;
   const double* __restrict__ Hq = P.H + (int64_t)(l + nb - 1) * ldH + l;
   // [W | Z] of the workgroup's SPW shifts, CW columns per chunk, in LDS as
@@ -321,9 +354,10 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   };
 
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
+  int E;   // scale exponent of the tile's y
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
     own_rows_update<RPL, 64, CW, BMAX, NL, 4>(P, J, it.k0, k, wave, l, nb, lane, mu, Hs, Bsh, pf,
-                                             v, y);
+                                             v, y, E);
   } else {       // the last tile column starts from the initial vectors
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
@@ -331,13 +365,13 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       v[q] = r < nb ? Rin[r] : 0.0;
       y[q] = r < nb ? yg[r] : 0.0;
     }
+    E = P.Ep[(J + 1) & 1][(int64_t)J * m + k];
   }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     cr[q] = 1.0;
     sr[q] = 0.0;
   }
-  int E = P.E[(int64_t)J * m + k];
   const double Hnorm = P.diagn[J];
   const double eps = 0x1p-53;
   const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
@@ -592,7 +626,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       }
     }
     if (lane == 0) {
-      P.E[(int64_t)J * m + k] = E;
+      P.Ef[(int64_t)J * m + k] = E;
       if (info) atomicOr(&P.status[k], 2);
     }
   }
@@ -745,9 +779,10 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   };
 
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
+  int E;   // scale exponent of the tile's y
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
     own_rows_update<RPL, 16, CW, BMAX, NL, 16>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
-                                               Bsh, pf, v, y);
+                                               Bsh, pf, v, y, E);
   } else {       // the last tile column starts from the initial vectors
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
@@ -755,13 +790,13 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       v[q] = r < nb ? Rin[r] : 0.0;
       y[q] = r < nb ? yg[r] : 0.0;
     }
+    E = P.Ep[(J + 1) & 1][(int64_t)J * m + k];
   }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     cr[q] = 1.0;
     sr[q] = 0.0;
   }
-  int E = P.E[(int64_t)J * m + k];
   const double Hnorm = P.diagn[J];
   const double eps = 0x1p-53;
   const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
@@ -1062,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       }
     }
     if (t == 0) {
-      P.E[(int64_t)J * m + k] = E;
+      P.Ef[(int64_t)J * m + k] = E;
       if (info) atomicOr(&P.status[k], 2);
     }
   }
@@ -1134,47 +1169,6 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       P.aux[J & 1][4 * k + 2] = zn;
       P.aux[J & 1][4 * k + 3] = (double)p;
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Scaling exponents of the panel pass of tile column J for every tile row
-// I < J: consolidate the scales of y_I and Z_J (update-task.c:68-88), then
-// one protection for the whole fused update (update-task.c:103-248):
-// |update(i)| <= (||H(I,:)||_oo + |mu|) ||Z|| + |R~(i)| |Zlast| and
-// |R~(i)| <= ||(H - mu I)(i,:)||_2, so t_I = 2 (||H(I,:)||_oo + |mu|).
-// Runs after the previous panel pass finished (it reads the row-block
-// norms that pass produced) and before this one.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_params(const ProbDev* __restrict__ probs,
-                                                const K1Item* __restrict__ items) {
-  __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
-  const K1Item it = items[blockIdx.x];
-  const ProbDev& P = probs[it.prob];
-  const int J = it.J;
-  if (J == 0) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int k = it.k0 + wave;
-  const int m = P.m;
-  if (k >= m) return;
-  const double mu = P.wr[k];
-  const double zn = P.aux[J & 1][4 * k + 2];
-  const int p = (int)P.aux[J & 1][4 * k + 3];
-  const int EJ = P.E[(int64_t)J * m + k];
-  for (int I = lane; I < J; I += 64) {
-    const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
-    double yn = 0.0;
-    for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, P.Ynb[(int64_t)rb * m + k]);
-    const int EI = P.E[(int64_t)I * m + k];
-    const int c0 = min(EI, EJ);
-    const double yI = ldexp(yn, c0 - EI);
-    const double zJ = ldexp(zn, c0 - EJ);
-    const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
-    const int ea = pow2_exp(protect_update(tI, zJ, yI));
-    const int ec = c0 + ea;
-    P.SY[(int64_t)I * m + k] = ec - EI;
-    P.SZ[(int64_t)I * m + k] = ec - EJ + p;
-    P.E[(int64_t)I * m + k] = ec;
   }
 }
 
@@ -1285,13 +1279,17 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
       const int k = col0 / 2 + tid;
       if (k < m) {
         const double* aux = P.aux[J & 1];
-        pmu[tid] = P.wr[k];
+        const double mu = P.wr[k];
+        pmu[tid] = mu;
         pP[tid] = aux[4 * k + 0];
         pZl[tid] = aux[4 * k + 1];
         pb0w[tid] = Bg[(int64_t)(2 * k) * b];
         pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
-        psy[tid] = P.SY[(int64_t)tile * m + k];
-        psz[tid] = P.SZ[(int64_t)tile * m + k];
+        int sy, sz;
+        const int ec = update_exponents(P, tile, J, k, mu, sy, sz);
+        psy[tid] = sy;
+        psz[tid] = sz;
+        if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
       }
     }
     stash(0);
@@ -1392,7 +1390,7 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
       const int cl = tid;
       if (cl < ncols && (cl & 1)) {
         const int k = (col0 + cl) >> 1;
-        P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+        P.Ynb[J & 1][(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
       }
     }
     if (!LOOP) break;
@@ -1492,13 +1490,17 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
       int ez = 0;
       if (k < m) {
         const double* aux = P.aux[J & 1];
-        pmu[tid] = P.wr[k];
+        const double mu = P.wr[k];
+        pmu[tid] = mu;
         pP[tid] = aux[4 * k + 0];
         pZl[tid] = aux[4 * k + 1];
         pb0w[tid] = Bg[(int64_t)(2 * k) * b];
         pb0z[tid] = Bg[(int64_t)(2 * k + 1) * b];
-        psy[tid] = P.SY[(int64_t)tile * m + k];
-        psz[tid] = P.SZ[(int64_t)tile * m + k];
+        int sy, sz;
+        const int ec = update_exponents(P, tile, J, k, mu, sy, sz);
+        psy[tid] = sy;
+        psz[tid] = sz;
+        if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
         const double zn = aux[4 * k + 2];   // ||Z|| before the prescale 2^-p
         if (zn > 0.0 && isfinite(zn)) ez = ilogb(zn) - (int)aux[4 * k + 3];
       }
@@ -1605,7 +1607,7 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
       const int cl = tid;
       if (cl < ncols && (cl & 1)) {
         const int k = (col0 + cl) >> 1;
-        P.Ynb[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+        P.Ynb[J & 1][(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
       }
     }
     if (!LOOP) break;
@@ -1647,7 +1649,7 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
 
   // smallest exponent over the tiles (reduce_scaling_factors, robust.c:58-82)
   int em = 0x7fffffff;
-  for (int t = tid; t < T; t += 256) em = min(em, P.E[(int64_t)t * m + k]);
+  for (int t = tid; t < T; t += 256) em = min(em, P.Ef[(int64_t)t * m + k]);
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) em = min(em, __shfl_xor(em, off, 64));
   if (lane == 0) emin_s[wave] = em;
@@ -1657,7 +1659,7 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
   double* x = P.Y + (int64_t)k * P.ldY;
   const double* c = P.C + (int64_t)k * n;
   const double* s = P.S + (int64_t)k * n;
-  const int* Ek = P.E + k;
+  const int* Ek = P.Ef + k;
 
   double carry = 0.0;   // tau1 entering the segment
   Acc3 acc = {0.0, 0.0, 0.0};
@@ -1970,12 +1972,6 @@ hipError_t launch_sweep(int pack, int b, const ProbDev* d_probs, const K1Item* i
                         hipStream_t st) {
   if (pack == 4) return launch_diag16(b / 16, d_probs, items, count, st);
   return launch_diag(b <= 64 ? 1 : (b <= 128 ? 2 : 4), d_probs, items, count, st);
-}
-
-hipError_t launch_params(const ProbDev* d_probs, const K1Item* items, int count, hipStream_t st) {
-  if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_params, dim3(count), dim3(256), 0, st, d_probs, items);
-  return hipGetLastError();
 }
 
 hipError_t launch_update(const ProbDev* d_probs, const GemmItem* items, int count,

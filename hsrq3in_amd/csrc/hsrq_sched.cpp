@@ -42,7 +42,8 @@ int choose_pack(int requested, int b, int total_shifts) {
 
 namespace {
 struct Lay {
-  size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb, rowinf, diagn, part, scal, e, sy, sz, st;
+  size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb0, ynb1, rowinf, diagn, part, scal, ep0, ep1,
+      ef, st;
   size_t total;
 };
 
@@ -67,14 +68,15 @@ Lay layout(int n, int m, int b) {
   L.bm1 = take((size_t)b * 2 * m * sizeof(double));
   L.aux0 = take((size_t)4 * m * sizeof(double));
   L.aux1 = take((size_t)4 * m * sizeof(double));
-  L.ynb = take((size_t)nrb * m * sizeof(double));
+  L.ynb0 = take((size_t)nrb * m * sizeof(double));
+  L.ynb1 = take((size_t)nrb * m * sizeof(double));
   L.rowinf = take((size_t)T * sizeof(double));
   L.diagn = take((size_t)T * sizeof(double));
   L.part = take((size_t)nrb * n * sizeof(double));
   L.scal = take(4 * sizeof(double));
-  L.e = take(Tm * sizeof(int));
-  L.sy = take(Tm * sizeof(int));
-  L.sz = take(Tm * sizeof(int));
+  L.ep0 = take(Tm * sizeof(int));
+  L.ep1 = take(Tm * sizeof(int));
+  L.ef = take(Tm * sizeof(int));
   L.st = take((size_t)std::max(m, 1) * sizeof(int));
   L.total = x;
   return L;
@@ -108,14 +110,15 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.Bm[1] = (double*)(ws + L.bm1);
   P.aux[0] = (double*)(ws + L.aux0);
   P.aux[1] = (double*)(ws + L.aux1);
-  P.Ynb = (double*)(ws + L.ynb);
+  P.Ynb[0] = (double*)(ws + L.ynb0);
+  P.Ynb[1] = (double*)(ws + L.ynb1);
   P.rowinf = (double*)(ws + L.rowinf);
   P.diagn = (double*)(ws + L.diagn);
   P.part = (double*)(ws + L.part);
   P.scal = (double*)(ws + L.scal);
-  P.E = (int*)(ws + L.e);
-  P.SY = (int*)(ws + L.sy);
-  P.SZ = (int*)(ws + L.sz);
+  P.Ep[0] = (int*)(ws + L.ep0);
+  P.Ep[1] = (int*)(ws + L.ep1);
+  P.Ef = (int*)(ws + L.ef);
   P.status = (int*)(ws + L.st);
 }
 
@@ -138,7 +141,6 @@ void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out) {
   const int k1step = pack == 4 ? 16 : kShiftsPerK1;
   for (int k0 = 0; k0 < P.m; k0 += k1step) out.k1.push_back({i, J, k0, 0});
   if (J == 0) return;
-  for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) out.pr.push_back({i, J, k0, 0});
   // Panel pass of the rows above tile J-1 (tile J-1's own rows are updated
   // at the start of its sweep), in row-block pairs, XCD-aware order: blocks
   // are dealt to the 8 XCDs round robin, so pairs 8t .. 8t+7 go out

@@ -334,7 +334,7 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   size_t need = align_up(part.size() * sizeof(GemmItem)) + align_up(fin.size() * sizeof(RowItem)) +
                 align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem));
   for (auto& kv : steps)
-    need += align_up(kv.second.k1.size() * sizeof(K1Item)) + align_up(kv.second.pr.size() * sizeof(K1Item)) +
+    need += align_up(kv.second.k1.size() * sizeof(K1Item)) +
             align_up(kv.second.gb.size() * sizeof(GemmItem));
   need = std::max(need, kAlign);
   if (need > s->cap_items[par]) {
@@ -368,9 +368,9 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   const ShiftItem* d_bt = (const ShiftItem*)put(bt.data(), bt.size() * sizeof(ShiftItem));
   struct Dev {
     int b, pack;
-    const K1Item *k1, *pr;
+    const K1Item* k1;
     const GemmItem* gb;
-    int nk1, npr, ngb;
+    int nk1, ngb;
   };
   std::vector<Dev> dev;
   for (auto& kv : steps) {
@@ -378,10 +378,8 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
     x.b = kv.first / 8;
     x.pack = kv.first % 8;
     x.k1 = (const K1Item*)put(kv.second.k1.data(), kv.second.k1.size() * sizeof(K1Item));
-    x.pr = (const K1Item*)put(kv.second.pr.data(), kv.second.pr.size() * sizeof(K1Item));
     x.gb = (const GemmItem*)put(kv.second.gb.data(), kv.second.gb.size() * sizeof(GemmItem));
     x.nk1 = (int)kv.second.k1.size();
-    x.npr = (int)kv.second.pr.size();
     x.ngb = (int)kv.second.gb.size();
     dev.push_back(x);
   }
@@ -411,8 +409,6 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   if ((e = launch_init(P, d_init, (int)init.size(), s->stream)) != hipSuccess) return fail(e);
   for (const Dev& x : dev)
     if ((e = launch_sweep(x.pack, x.b, P, x.k1, x.nk1, s->stream)) != hipSuccess) return fail(e);
-  for (const Dev& x : dev)
-    if ((e = launch_params(P, x.pr, x.npr, s->stream)) != hipSuccess) return fail(e);
   for (const Dev& x : dev)
     if ((e = launch_update(P, x.gb, x.ngb, s->stream)) != hipSuccess) return fail(e);
   if ((e = launch_backtransform(P, d_bt, (int)bt.size(), s->stream)) != hipSuccess) return fail(e);

@@ -116,9 +116,10 @@ int hsrq_plan_execute(hsrq_plan *plan);
 int hsrq_plan_execute_graph(hsrq_plan *plan);
 /* Synchronous execution with HIP events around every launch:
  * kernel_ms[6] / launches[6] receive the summed device time and launch count
- * per kernel kind: 0 norms, 1 init, 2 diagonal sweep, 3 panel update,
- * 4 back-transform, 5 update scaling exponents.  Kernels of the two streams
- * overlap, so the sum can exceed the wall time. */
+ * per kernel kind: 0 norms, 1 init, 2 diagonal sweep (with the panel pass of
+ * its own tile's rows), 3 panel pass of the rows above, 4 back-transform,
+ * 5 reserved (0).  Kernels of the two streams overlap, so the sum can exceed
+ * the wall time. */
 int hsrq_plan_execute_profiled(hsrq_plan *plan, double *kernel_ms, int *launches);
 int hsrq_plan_sync(hsrq_plan *plan);
 /* Copies the per-shift status bits (sum(m) ints) to host. */

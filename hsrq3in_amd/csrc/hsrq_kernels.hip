@@ -218,8 +218,6 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
   const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
   const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
   const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
-  int sy, sz;
-  E0 = update_exponents(P, J, J + 1, k, mu, sy, sz);
   const int64_t ldH = P.ldH;
   const double* __restrict__ Hq = P.H + (int64_t)(l + nb - 1) * ldH + l;
   // [W | Z] of the workgroup's SPW shifts, CW columns per chunk, in LDS as
@@ -249,6 +247,8 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
   }
   const int nch = b / CW;
   stage(0);
+  int sy, sz;   // after the first chunk's loads are issued: the latencies overlap
+  E0 = update_exponents(P, J, J + 1, k, mu, sy, sz);
   for (int ch = 0; ch < nch; ++ch) {
     __syncthreads();
 #pragma unroll
@@ -261,15 +261,32 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
     }
     __syncthreads();
     if (ch + 1 < nch) stage(ch + 1);
-#pragma unroll 8
-    for (int cc = 0; cc < CW; ++cc) {
-      const double wv = Bsh[cc * 2 * SPW + 2 * sidx], zv = Bsh[cc * 2 * SPW + 2 * sidx + 1];
+    // G columns per group, the next group's LDS reads issued before this
+    // group's FMAs (software pipelined; the column order of the sums is kept).
+    // G = 1 where registers are short (the packed layouts).
+    constexpr int G = RPL == 2 ? 4 : (RPL == 1 ? 2 : 1);
+    double hb[2][G][RPL], wb[2][G], zb[2][G];
+    auto ld = [&](int buf, int c0) __attribute__((always_inline)) {
 #pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        const double h = Hs[cc * BMAX + r0 + STRIDE * q];
-        aw[q] = fma(h, wv, aw[q]);
-        az[q] = fma(h, zv, az[q]);
+      for (int g = 0; g < G; ++g) {
+        wb[buf][g] = Bsh[(c0 + g) * 2 * SPW + 2 * sidx];
+        zb[buf][g] = Bsh[(c0 + g) * 2 * SPW + 2 * sidx + 1];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) hb[buf][g][q] = Hs[(c0 + g) * BMAX + r0 + STRIDE * q];
       }
+    };
+    ld(0, 0);
+#pragma unroll
+    for (int c0 = 0; c0 < CW; c0 += G) {
+      const int cur = (c0 / G) & 1;
+      if (c0 + G < CW) ld(cur ^ 1, c0 + G);
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+          aw[q] = fma(hb[cur][g][q], wb[cur][g], aw[q]);
+          az[q] = fma(hb[cur][g][q], zb[cur][g], az[q]);
+        }
     }
   }
   __syncthreads();   // the chunk buffer is the sweep's again

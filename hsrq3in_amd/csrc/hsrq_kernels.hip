@@ -1253,13 +1253,13 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
     double pa[NPA], pb[NPB];
     auto fetch = [&](int kc) {
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
         pa[i] = (rr < nr) ? Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH] : 0.0;
       }
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
@@ -1267,13 +1267,13 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
       }
     };
     auto stash = [&](int buf) {
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
         As[buf][kk * LDA64 + rr] = pa[i];
       }
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
@@ -1285,9 +1285,9 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     // values; the result D^T[col][row] sits in register col/4 of lane
     // row + 16 (col % 4): one lane = one matrix row, four columns per tile.
     double4_t acc[2][CT];
-  #pragma unroll
+#pragma unroll
     for (int a = 0; a < 2; ++a)
-  #pragma unroll
+#pragma unroll
       for (int c = 0; c < CT; ++c) acc[a][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
     const int nchunks = b / KC;
@@ -1312,8 +1312,8 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     stash(0);
     __syncthreads();
 
-    // Epilogue operands (R~ in, y), loaded while the last K chunk runs on
-    // MFMA: acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
+    // Epilogue operands (R~ in, y), one 16-column group at a time:
+    // acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
     // wn*GN/2 + c*16 + 4*r + lane/16; even lane groups hold W columns (R~
     // out), odd ones Z columns (y).
     const double* __restrict__ Rin = P.Rt[J & 1];
@@ -1323,12 +1323,12 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     const int rl = lane & 15;
     double rin[4][2], yv[4][2];   // one 16-column group at a time (registers)
     auto load_epilogue = [&](int c) __attribute__((always_inline)) {
-  #pragma unroll
+#pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
           const int k = (col0 + cl) >> 1;
           const bool okc = cl < ncols;
-  #pragma unroll
+#pragma unroll
           for (int a = 0; a < 2; ++a) {
             const int row = wm * 32 + a * 16 + rl;
             const int gi = r0 + row;
@@ -1339,18 +1339,18 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
         }
     };
     auto mfma_chunk = [&](int buf) __attribute__((always_inline)) {
-  #pragma unroll
+#pragma unroll
       for (int k4 = 0; k4 < KC; k4 += 4) {
         const int kr = k4 + (lane >> 4);
         double hv[2], zv[CT];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA64 + wm * 32 + a * 16 + (lane & 15)];
-  #pragma unroll
+#pragma unroll
         for (int c = 0; c < CT; ++c)
           zv[c] = Bs[buf][(wn * (GN / 2) + c * 16 + (lane & 15)) * LDB64 + kr];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 2; ++a)
-  #pragma unroll
+#pragma unroll
           for (int c = 0; c < CT; ++c)
             acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(zv[c], hv[a], acc[a][c], 0, 0, 0);
       }
@@ -1366,10 +1366,10 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
     __syncthreads();   // As is reused for the column maxima below
 
     // Epilogue, one 16-column group at a time.
-  #pragma unroll
+#pragma unroll
     for (int c = 0; c < CT; ++c) {
       load_epilogue(c);
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int cl = wn * (GN / 2) + c * 16 + 4 * r + g;
         const int ks = cl >> 1;
@@ -1381,7 +1381,7 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
           const double coef = isz ? pZl[ks] : pP[ks];
           const double b0 = isz ? pb0z[ks] : pb0w[ks];
           const int sy = psy[ks], sz = psz[ks];
-  #pragma unroll
+#pragma unroll
           for (int a = 0; a < 2; ++a) {
             const int row = wm * 32 + a * 16 + rl;
             const int gi = r0 + row;
@@ -1465,13 +1465,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
     float pa[NPA], pb[NPB];
     auto fetch = [&](int kc) {
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
         pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH], -eH) : 0.0f;
       }
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
@@ -1481,13 +1481,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
       }
     };
     auto stash = [&](int buf) {
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
         As[buf][kk * LDA + rr] = pa[i];
       }
-  #pragma unroll
+#pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
@@ -1496,9 +1496,9 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     };
 
     float4_t acc[2][CT];
-  #pragma unroll
+#pragma unroll
     for (int a = 0; a < 2; ++a)
-  #pragma unroll
+#pragma unroll
       for (int c = 0; c < CT; ++c) acc[a][c] = (float4_t){0.f, 0.f, 0.f, 0.f};
 
     const int nchunks = b / KC;
@@ -1530,18 +1530,18 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     for (int ch = 0; ch < nchunks; ++ch) {
       const int buf = ch & 1;
       if (ch + 1 < nchunks) fetch((ch + 1) * KC);
-  #pragma unroll
+#pragma unroll
       for (int k4 = 0; k4 < KC; k4 += 4) {
         const int kr = k4 + (lane >> 4);
         float hv[2], zv[CT];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 2; ++a) hv[a] = As[buf][kr * LDA + wm * 32 + a * 16 + (lane & 15)];
-  #pragma unroll
+#pragma unroll
         for (int c = 0; c < CT; ++c)
           zv[c] = Bs[buf][kr * LDB + wn * (GN / 2) + c * 16 + (lane & 15)];
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 2; ++a)
-  #pragma unroll
+#pragma unroll
           for (int c = 0; c < CT; ++c)
             acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(zv[c], hv[a], acc[a][c], 0, 0, 0);
       }
@@ -1556,14 +1556,14 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     const int g = lane >> 4;
     const int rl = lane & 15;
     double rin[CT][4][2], yv[CT][4][2];
-  #pragma unroll
+#pragma unroll
     for (int c = 0; c < CT; ++c)
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
         const int k = (col0 + cl) >> 1;
         const bool okc = cl < ncols;
-  #pragma unroll
+#pragma unroll
         for (int a = 0; a < 2; ++a) {
           const int row = wm * 32 + a * 16 + rl;
           const int gi = r0 + row;
@@ -1573,13 +1573,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
         }
       }
     double cmaxs[CT][2];
-  #pragma unroll
+#pragma unroll
     for (int c = 0; c < CT; ++c)
-  #pragma unroll
+#pragma unroll
       for (int h = 0; h < 2; ++h) cmaxs[c][h] = 0.0;
-  #pragma unroll
+#pragma unroll
     for (int c = 0; c < CT; ++c)
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
         const bool isz = (r & 1) != 0;
         const int cl = wn * (GN / 2) + c * 16 + 4 * g + r;
@@ -1592,7 +1592,7 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
           const double b0 = isz ? pb0z[ks] : pb0w[ks];
           const int sy = psy[ks], sz = psz[ks];
           const int es = eH + (isz ? pez[ks] : 0);
-  #pragma unroll
+#pragma unroll
           for (int a = 0; a < 2; ++a) {
             const int row = wm * 32 + a * 16 + rl;
             const int gi = r0 + row;
@@ -1611,9 +1611,9 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
         }
       }
     // per-column maxima of |y| over the block's 64 rows (Z columns only)
-  #pragma unroll
+#pragma unroll
     for (int c = 0; c < CT; ++c)
-  #pragma unroll
+#pragma unroll
       for (int h = 0; h < 2; ++h) {
         const double x = row16_max(cmaxs[c][h]);
         const int cl = wn * (GN / 2) + c * 16 + 4 * g + 2 * h + 1;

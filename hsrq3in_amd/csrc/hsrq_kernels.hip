@@ -795,7 +795,12 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
     }
   };
 
-  double v[RPL], y[RPL], cr[RPL], sr[RPL];
+  double v[RPL], y[RPL];
+  // rotations go to C, S as they are made (the lane owning the row stores
+  // them), not kept in registers: 2 RPL registers and the selects of a
+  // register-array update per step less
+  double* __restrict__ Cg = P.C + (int64_t)k * n + l;
+  double* __restrict__ Sg = P.S + (int64_t)k * n + l;
   int E;   // scale exponent of the tile's y
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
     own_rows_update<RPL, 16, CW, BMAX, NL, 16>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
@@ -808,11 +813,6 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       y[q] = r < nb ? yg[r] : 0.0;
     }
     E = P.Ep[(J + 1) & 1][(int64_t)J * m + k];
-  }
-#pragma unroll
-  for (int q = 0; q < RPL; ++q) {
-    cr[q] = 1.0;
-    sr[q] = 0.0;
   }
   const double Hnorm = P.diagn[J];
   const double eps = 0x1p-53;
@@ -948,11 +948,9 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
     By = ldexp(a, kk + e2) * slack;
   };
   auto keep_rotation = [&](int st, const VOut& o) __attribute__((always_inline)) {
-    const int own = (t == (st & 15)) ? (st >> 4) : -1;
-#pragma unroll
-    for (int q = 0; q < RPL; ++q) {
-      cr[q] = (own == q) ? o.c : cr[q];
-      sr[q] = (own == q) ? o.s : sr[q];
+    if (valid && t == (st & 15)) {
+      Cg[st] = o.c;
+      Sg[st] = o.s;
     }
   };
 
@@ -1080,10 +1078,13 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       double c0 = 1.0, s0 = 0.0;
       if (hleft != 0.0) givens(v0, hleft, c0, s0);
       v0 = v0 * c0 - hleft * s0;
-      if (t == 0) {
-        cr[0] = c0;
-        sr[0] = s0;
+      if (valid && t == 0) {
+        Cg[0] = c0;
+        Sg[0] = s0;
       }
+    } else if (valid && t == 0) {   // tile 0: row 0 is not rotated
+      Cg[0] = 1.0;
+      Sg[0] = 0.0;
     }
     double y0 = bcast16<0>(y[0]);
     double csr = v0;
@@ -1102,16 +1103,10 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   }
 
   if (valid) {
-    double* Cg = P.C + (int64_t)k * n + l;
-    double* Sg = P.S + (int64_t)k * n + l;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = t + 16 * q;
-      if (r < nb) {
-        yg[r] = y[q];
-        Cg[r] = cr[q];
-        Sg[r] = sr[q];
-      }
+      if (r < nb) yg[r] = y[q];
     }
     if (t == 0) {
       P.Ef[(int64_t)J * m + k] = E;
@@ -1125,12 +1120,16 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   double w[RPL], z[RPL];
   double carry_p = 1.0, carry_t = 0.0;
   double zmax = 0.0;
+  // the rotations this wave stored (other lanes of the same shift): visible
+  // to its loads once the stores completed (one CU, workgroup scope)
+  __threadfence_block();
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int r = t + 16 * q;
     const bool in = r < nb;
-    const double sq = in ? sr[q] : 1.0;
-    const double cq = in ? cr[q] : 0.0;
+    const bool ld = valid && in;   // a shift past m stored nothing: its results are dropped
+    const double sq = in ? (ld ? Sg[r] : 1.0) : 1.0;
+    const double cq = in ? (ld ? Cg[r] : 0.0) : 0.0;
     double inc = sq;
 #pragma unroll
     for (int off = 1; off < 16; off <<= 1) {

@@ -38,6 +38,10 @@ K_UPDATE_TRAFFIC = {
     (8000, 800, 128): (141.09e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
 }
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_diag launch (same PMC runs), by (n, m, tile).
+K_DIAG_TRAFFIC = {
+    (8000, 800, 128): (10.05e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
+}
 
 
 def update_flops(n: int, m: int, b: int) -> float:
@@ -50,6 +54,23 @@ def update_flops(n: int, m: int, b: int) -> float:
     for J in range(2, T):
         r = b0 + (J - 2) * b
         tot += 4.0 * r * b * m
+    return tot
+
+
+def sweep_bytes(n: int, m: int, b: int, shifts_per_group: int) -> float:
+    """Algorithmic HBM bytes of the k_diag launches of one solve: per tile J
+    the diagonal tile (8 nb^2) and, below the last tile column, the panel
+    block of its own-rows update (8 nb b) once per group of shifts that share
+    them (a workgroup), plus 24 nb m (y in and out, the emitted W/Z)."""
+    T = (n + b - 1) // b
+    b0 = n - (T - 1) * b
+    groups = -(-m // shifts_per_group)
+    tot = 0.0
+    for J in range(T):
+        nb = b0 if J == 0 else b
+        tot += 8.0 * nb * nb * groups + 24.0 * nb * m
+        if J < T - 1:
+            tot += 8.0 * nb * b * groups
     return tot
 
 
@@ -413,6 +434,19 @@ def main():
             "kernel": "k_update (FP64 MFMA panel pass)",
             "launches": upd_n, "avg_launch_us": round(1e3 * upd_ms / max(upd_n, 1), 2)}
     kernels = {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}
+    # the sweeps against the HBM roofline (they are bound by the latency of
+    # their rotation chain; this shows how far from bandwidth they run)
+    dg_ms, dg_n = prof["diag"]
+    spw = 16 if plan.lanes()[0][1] == 4 else 4
+    sb = sweep_bytes(n, m, tile, spw)
+    dtraffic = K_DIAG_TRAFFIC.get((n, m, tile))
+    sweep_gbs = sb / (dg_ms / 1e3) / 1e9 if dg_ms > 0 else 0.0
+    roof_sweep = {"bound": "hbm", "achieved": round(sweep_gbs, 1), "peak": PEAK_HBM_GBS,
+                  "unit": "GB/s", "frac": round(sweep_gbs / PEAK_HBM_GBS, 4),
+                  "traffic": dtraffic[0] if dtraffic else None,
+                  "algorithmic_bytes_per_launch": round(sb / max(dg_n, 1)),
+                  "kernel": "k_diag (sweep with its own-rows update)", "launches": dg_n,
+                  "avg_launch_us": round(1e3 * dg_ms / max(dg_n, 1), 2)}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "eigvecs/s", "n_gpus": 1,
@@ -427,7 +461,7 @@ def main():
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 4)},
         "converged": int(np.sum((st & HSRQ_CONVERGED) != 0)), "residual_ok": good,
         "residual_max": float(np.max(res)), "finite": finite,
-        "kernels": kernels, "roofline": roof,
+        "kernels": kernels, "roofline": roof, "roofline_sweep": roof_sweep,
     }
     plan.close()
     print(f"[bench] plain: {value:.1f} eigvecs/s", file=sys.stderr, flush=True)

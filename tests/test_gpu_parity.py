@@ -165,6 +165,36 @@ def test_lookahead_agrees(ctx, blocks_per_cu):
     assert (st2 & HSRQ_CONVERGED).all() and res.max() <= 10 * 3000
 
 
+def test_lookahead_batch_in_one_lane():
+    """Problems of different orders in one lane advance in lockstep (each at
+    its own tile column); with the lookahead each sweep waits only for the
+    panel pass two steps back.  Bitwise the same as the default plan."""
+    from hsrq3in_amd import Context, colmajor_to_device
+    cases = [case(n, seed=500 + n, m=m) for n, m in [(300, 20), (700, 40), (1200, 60), (450, 25)]]
+    def items():
+        return [(colmajor_to_device(H), torch.from_numpy(wr.copy()).cuda(),
+                 torch.zeros((len(wr), H.shape[0]), dtype=torch.float64, device="cuda"))
+                for H, wr in cases]
+    outs = []
+    for lanes, look in [(0, 0), (1, 2), (1, 1)]:
+        c = Context(0)
+        if lanes:
+            c.set_lanes(lanes)
+        c.set_lookahead(look)
+        a = items()
+        p = c.plan(a, 128)
+        p.execute(graph=True)
+        p.sync()
+        st = p.status()
+        p.close()
+        c.close()
+        assert (st & HSRQ_CONVERGED).all()
+        outs.append([v.cpu().numpy() for _, _, v in a])
+    for other in outs[1:]:
+        for xa, xb in zip(outs[0], other):
+            assert np.array_equal(xa, xb)
+
+
 def test_autotune_choice_is_consistent():
     """The autotuner picks a supported configuration and a plan built after
     it gives the same vectors as an explicit request of that configuration."""

@@ -184,14 +184,10 @@ void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, in
   }
 }
 
-// Workgroups per CU of the capped panel pass beside a diagonal sweep: the
-// room a sweep workgroup leaves (per SIMD 512 VGPRs, per CU 160 KiB LDS).
-// The looping panel pass takes 136 VGPRs per wave and 40 KiB of LDS per
-// block; k_diag with 1-2 rows per lane takes up to 224 VGPRs and 64 KiB,
-// the other sweep variants up to 256 VGPRs.  Measured, n = 8000, m = 800,
-// tile 128: 9.16 ms without lookahead, 8.83 / 7.60 / 8.87 / 8.58 ms with
-// 1 / 2 / 3 / 4 blocks per CU (MI355X).
-int auto_lookahead(int pack, int b) { return pack == 1 && b <= 128 ? 2 : 1; }
+// Workgroups per CU of the capped panel pass beside a diagonal sweep
+// (lookahead_blocks_per_cu).  Measured, n = 8000, m = 800, tile 128: 9.16 ms
+// without lookahead, 8.83 / 7.60 / 8.87 / 8.58 ms with 1 / 2 / 3 / 4 blocks
+// per CU (MI355X).
 
 // Item tables of one lane, uploaded once.  `single`: the plan has one lane,
 // which then may take a second stream for the panel pass of the rows above
@@ -254,7 +250,9 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
   if (cnt > 0)
     HIP_TRY(hipMemcpy(L->d_probs, L->probs.data(), cnt * sizeof(ProbDev), hipMemcpyHostToDevice));
   if (L->own_stream) HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-  const int la = !single ? 0 : ctx->lookahead < 0 ? auto_lookahead(L->pack, L->b) : ctx->lookahead;
+  const int la = !single             ? 0
+                 : ctx->lookahead < 0 ? lookahead_blocks_per_cu(L->pack, L->b)
+                                      : ctx->lookahead;
   if (la > 0) {
     HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
     L->side_blocks = la * std::max(ctx->cus, 1);

@@ -52,6 +52,9 @@ int validate_problem(const hsrq_problem& q);
 int choose_tile(const hsrq_context* ctx, int tlsz, int n);
 // Diagonal-sweep layout: 1 or 4 shifts per wavefront.
 int choose_pack(int requested, int b, int total_shifts);
+// Workgroups per CU of a panel pass that runs beside the diagonal sweeps of
+// this layout and tile: the most that leave room for a sweep workgroup.
+int lookahead_blocks_per_cu(int pack, int b);
 
 // Device workspace of one problem (bytes, 256-aligned pieces).
 size_t prob_workspace_bytes(int n, int m, int b);

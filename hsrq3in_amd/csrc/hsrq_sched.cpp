@@ -40,6 +40,11 @@ int choose_pack(int requested, int b, int total_shifts) {
   return total_shifts >= kPackedMinShifts ? 4 : 1;
 }
 
+// Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 136
+// VGPRs per wave and 40 KiB of LDS per block; k_diag with 1-2 rows per lane
+// up to 224 VGPRs and 68 KiB, the other sweep variants up to 256 VGPRs.
+int lookahead_blocks_per_cu(int pack, int b) { return pack == 1 && b <= 128 ? 2 : 1; }
+
 namespace {
 struct Lay {
   size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb0, ynb1, rowinf, diagn, part, scal, ep0, ep1,

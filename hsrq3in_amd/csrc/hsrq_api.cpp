@@ -253,7 +253,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
   const int la = !single             ? 0
                  : ctx->lookahead < 0 ? lookahead_blocks_per_cu(L->pack, L->b)
                                       : ctx->lookahead;
-  if (la > 0) {
+  if (la > 0) {   // 0: off (asked for, or the automatic choice for packed sweeps)
     HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
     L->side_blocks = la * std::max(ctx->cus, 1);
   }

@@ -53,7 +53,8 @@ int choose_tile(const hsrq_context* ctx, int tlsz, int n);
 // Diagonal-sweep layout: 1 or 4 shifts per wavefront.
 int choose_pack(int requested, int b, int total_shifts);
 // Workgroups per CU of a panel pass that runs beside the diagonal sweeps of
-// this layout and tile: the most that leave room for a sweep workgroup.
+// this layout and tile, automatic choice: the most that leave room for a
+// sweep workgroup for one shift per wavefront; 0 (no lookahead) for four.
 int lookahead_blocks_per_cu(int pack, int b);
 
 // Device workspace of one problem (bytes, 256-aligned pieces).

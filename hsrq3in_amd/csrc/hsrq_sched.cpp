@@ -41,9 +41,15 @@ int choose_pack(int requested, int b, int total_shifts) {
 }
 
 // Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 136
-// VGPRs per wave and 40 KiB of LDS per block; k_diag with 1-2 rows per lane
-// up to 224 VGPRs and 68 KiB, the other sweep variants up to 256 VGPRs.
-int lookahead_blocks_per_cu(int pack, int b) { return pack == 1 && b <= 128 ? 2 : 1; }
+// VGPRs per wave and 40 KiB of LDS per block; k_diag takes up to 224 VGPRs
+// and 68 KiB, so two panel-pass blocks fit beside it.  The packed sweeps run
+// where many shifts fill the GPU by themselves: there a capped panel pass
+// only slows the plan (256-problem batch in one lane: 31.4 ms without, 41.7
+// ms with one block per CU), so the automatic choice is no lookahead.
+int lookahead_blocks_per_cu(int pack, int b) {
+  (void)b;
+  return pack == 1 ? 2 : 0;
+}
 
 namespace {
 struct Lay {

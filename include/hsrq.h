@@ -64,10 +64,11 @@ int hsrq_context_set_packing(hsrq_context *ctx, int shifts_per_wave);
 int hsrq_context_set_lanes(hsrq_context *ctx, int lanes);
 /* k in 1..8: run the panel pass of the rows above the next diagonal tile on
  * a second stream, at most k workgroups per CU, overlapping the next tile's
- * panel pass and diagonal sweep; 0: off; -1: automatic (default: on, with
- * the most workgroups per CU that leave room for a sweep workgroup).
- * Applies to plans of one lane; plans of several lanes ignore it.  Results
- * do not depend on it. */
+ * panel pass and diagonal sweep; 0: off; -1: automatic (default: on with
+ * one shift per wavefront in the sweep, with the most workgroups per CU
+ * that leave room for a sweep workgroup; off with four, where the shifts
+ * fill the GPU by themselves).  Applies to plans of one lane; plans of
+ * several lanes ignore it.  Results do not depend on it. */
 int hsrq_context_set_lookahead(hsrq_context *ctx, int blocks_per_cu);
 /* Panel-pass precision: HSRQ_PREC_FP64 (default), HSRQ_PREC_FP32 or
  * HSRQ_PREC_FP32_RAW.  In the FP32 modes the panel GEMM runs in FP32 on MFMA

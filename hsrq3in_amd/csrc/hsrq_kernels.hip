@@ -224,12 +224,17 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
   // Bsh[c * 2 SPW + 2 s + {0: W, 1: Z}]
   constexpr int NBT = CW * 2 * SPW, NB = (NBT + 255) / 256;
   double pb[NB];
+  constexpr int CSTEP = 256 / BMAX;   // thread t: row t % BMAX, columns t / BMAX + CSTEP i
+  const int hr = threadIdx.x % BMAX, hcb = threadIdx.x / BMAX;
+  const int64_t hstep = (int64_t)CSTEP * ldH;
   auto stage = [&](int chunk) __attribute__((always_inline)) {
+    const double* __restrict__ src = Hq + hr + (int64_t)(chunk * CW + hcb) * ldH;
+    if (nb == BMAX) {   // full tile
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int c = idx / BMAX, r = idx - c * BMAX;
-      pf[i] = r < nb ? Hq[r + (int64_t)(chunk * CW + c) * ldH] : 0.0;
+      for (int i = 0; i < NL; ++i) pf[i] = src[i * hstep];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) pf[i] = hr < nb ? src[i * hstep] : 0.0;
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
@@ -359,14 +364,21 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   constexpr int NL = CW * BMAX / 256;
   double pf[NL];
 
+  // thread t loads row t % BMAX of columns t / BMAX + CSTEP i of the chunk
+  constexpr int CSTEP = 256 / BMAX;
+  static_assert(256 % BMAX == 0 && CW % CSTEP == 0, "chunk tiling");
   auto issue = [&](int chunk) {
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
+    const int r = threadIdx.x % BMAX, cb = threadIdx.x / BMAX;
+    const double* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
+    const int64_t step = (int64_t)CSTEP * ldH;
+    if (nc == CW && nb == BMAX) {   // full chunk of a full tile
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int c = idx / BMAX, r = idx - c * BMAX;
-      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+      for (int i = 0; i < NL; ++i) pf[i] = src[i * step];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) pf[i] = (cb + CSTEP * i < nc && r < nb) ? src[i * step] : 0.0;
     }
   };
 
@@ -784,14 +796,21 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   constexpr int NL = CW * BMAX / 256;
   double pf[NL];
 
+  // thread t loads row t % BMAX of columns t / BMAX + CSTEP i of the chunk
+  constexpr int CSTEP = 256 / BMAX;
+  static_assert(256 % BMAX == 0 && CW % CSTEP == 0, "chunk tiling");
   auto issue = [&](int chunk) __attribute__((always_inline)) {
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
+    const int r = threadIdx.x % BMAX, cb = threadIdx.x / BMAX;
+    const double* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
+    const int64_t step = (int64_t)CSTEP * ldH;
+    if (nc == CW && nb == BMAX) {   // full chunk of a full tile
 #pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int c = idx / BMAX, r = idx - c * BMAX;
-      pf[i] = (c < nc && r < nb) ? H[(l + r) + (int64_t)(l + c0 + c) * ldH] : 0.0;
+      for (int i = 0; i < NL; ++i) pf[i] = src[i * step];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) pf[i] = (cb + CSTEP * i < nc && r < nb) ? src[i * step] : 0.0;
     }
   };
 

@@ -5,9 +5,11 @@ residual ||(H - mu I) x|| / (||H||_oo ||x|| 2^-52) stays <= 10 n (vectors
 whose FP32 result fails it are solved again on the FP64 path) and 1 - |cos|
 against the FP64 path stays <= 1e-10 on Gaussian matrices and prescribed
 spectra.  Graded matrices (row norms spread over more than 2^20) are solved
-on the FP64 path in HSRQ_PREC_FP32 mode, bit-identical to HSRQ_PREC_FP64;
-the raw FP32 pass loses their small vector components (unseen by the
-residual relative to ||H||_oo), so it is held only to finiteness there.
+on the FP64 path in HSRQ_PREC_FP32 mode, bit-identical to HSRQ_PREC_FP64.
+The raw FP32 pass is held to the same 1 - |cos| <= 1e-10 on every kind,
+graded ones included: the rows of the next diagonal tile are updated in
+FP64 at the start of its sweep in every mode, so the FP32 error reaches the
+rotations only through the damped cross-over terms.
 """
 import numpy as np
 import pytest
@@ -56,8 +58,9 @@ def test_fp32_against_fp64(kind, n):
     print(f"{kind} n={n}: raw 1-|cos| {e_raw:.2e} resid {rr.max():.2e}; "
           f"checked {e_cor:.2e} resid {r32.max():.2e}; fp64 resid {r64.max():.2e}")
     assert r32.max() <= 10 * n
+    assert e_raw <= 1e-10
     if kind.startswith("graded"):
         assert np.array_equal(x32, x64)
     else:
-        assert e_raw <= 1e-10 and e_cor <= 1e-10
+        assert e_cor <= 1e-10
         assert np.all(r32 <= np.maximum(rr, 10 * n))

@@ -41,7 +41,7 @@ int choose_pack(int requested, int b, int total_shifts) {
 }
 
 // Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 136
-// VGPRs per wave and 40 KiB of LDS per block; k_diag takes up to 224 VGPRs
+// VGPRs per wave and 40 KiB of LDS per block; k_diag takes up to 200 VGPRs
 // and 68 KiB, so two panel-pass blocks fit beside it.  The packed sweeps run
 // where many shifts fill the GPU by themselves: there a capped panel pass
 // only slows the plan (256-problem batch in one lane: 31.4 ms without, 41.7

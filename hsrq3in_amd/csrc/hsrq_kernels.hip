@@ -768,12 +768,15 @@ __device__ __forceinline__ double bc16(double x, int lrt) {
 template <int V>
 using IC = std::integral_constant<int, V>;
 
+// Three wavefronts per SIMD (at most 168 VGPRs) where the rows per lane allow
+// it: the packed sweeps run where many shifts are in flight, and their rate
+// is set by how many wavefronts hide each other's latency.
 template <int RPL>
-__global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev* __restrict__ probs,
                                                 const K1Item* __restrict__ items) {
   __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   constexpr int BMAX = 16 * RPL;
-  constexpr int CW = (BMAX <= 64) ? 64 : (BMAX <= 128 ? 32 : 16);   // LDS chunk columns
+  constexpr int CW = (BMAX <= 64) ? 64 : 16;   // LDS chunk columns (registers of the staging)
   __shared__ double Hs[CW * BMAX];
   __shared__ double Bsh[CW * 2 * 16];   // [W | Z] chunk of the 16 shifts
 
@@ -867,11 +870,17 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   }
 
   struct VOut {
-    double c, s, r, rinv, br, bv;
+    double c, s, r, rinv, br, bv, hdd;
   };
-  // v-chain step st (see k_diag); Lc: compile-time lane of row st, or -1.
-  auto vstep = [&](auto Lc, int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
-                   double (&nv)[RPL]) __attribute__((always_inline)) -> bool {
+  // v-chain step st (see k_diag), in two parts: the scalar rotation (vscal,
+  // returns the rare flag; nothing changed yet) and the vector work (vvec:
+  // column st of R into nr, the running column updated in place).  Lc:
+  // compile-time lane of row st, or -1.  The y chain likewise (yscal, yvec).
+  // Every element gets the operations of k_diag in the same order; with the
+  // rare cases decided before any vector work, no step needs copies of v, y
+  // or R to fall back on (registers for 3 wavefronts per SIMD).
+  auto vscal = [&](auto Lc, int st, const double* hc, bool full, VOut& o)
+      __attribute__((always_inline)) -> bool {
     constexpr int L = decltype(Lc)::value;
     const double vs = bc16<L>(pick<RPL>(v, st >> 4), st & 15);
     const double hsub = hc[st];
@@ -900,31 +909,38 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
     o.rinv = rinv;
     o.br = fma(fabs(c), Bv, fabs(s) * Hm) * slack;
     o.bv = fma(fabs(c), Hm, fabs(s) * Bv) * slack;
+    o.hdd = hdd;
+    return rare;
+  };
+  auto vvec = [&](int st, const double* hc, const VOut& o, double (&nr)[RPL])
+      __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = t + 16 * q;
       const bool act = row <= st - 1;
-      const double h = (row == st - 1) ? hdd : hc[row];
-      nr[q] = act ? fma(c, v[q], -(s * h)) : 0.0;
-      nv[q] = act ? fma(c, h, s * v[q]) : v[q];
+      const double h = (row == st - 1) ? o.hdd : hc[row];
+      const double vq = v[q];
+      nr[q] = act ? fma(o.c, vq, -(o.s * h)) : 0.0;
+      v[q] = act ? fma(o.c, h, o.s * vq) : vq;
     }
-    return rare;
   };
-  auto ystep_fast = [&](auto Lc, int st, const VOut& o, const double (&r)[RPL],
-                        double (&ny)[RPL], double& byn) __attribute__((always_inline)) -> bool {
+  auto yscal = [&](auto Lc, int st, const VOut& o, double& yq, double& byn)
+      __attribute__((always_inline)) -> bool {
     constexpr int L = decltype(Lc)::value;
     const double yj = bc16<L>(pick<RPL>(y, st >> 4), st & 15);
     const double acsr = fabs(o.r);
     bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
-    const double yq = yj * o.rinv;
+    yq = yj * o.rinv;
     byn = fma(fabs(yq), o.br, By) * slack;
     rare = rare | !(byn <= kOmega);
+    return rare;
+  };
+  auto yvec = [&](int st, double yq, const double (&r)[RPL]) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = t + 16 * q;
-      ny[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
+      y[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
     }
-    return rare;
   };
   auto ystep_slow = [&](int st, const VOut& o, const double (&r)[RPL]) __attribute__((always_inline)) {
     double yj = bcast16_rt(pick<RPL>(y, st >> 4), st & 15);
@@ -976,16 +992,11 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
   if (nb >= 2) {
     issue((nb - 2) / CW);
     VOut cv;
-    double rr[RPL], nr[RPL], nv[RPL], ny[RPL];
+    double rr[RPL];   // column of R of the pending y step
     {
       const double* hc = column(nb - 2);
-      if (vstep(IC<-1>{}, nb - 1, hc, false, cv, nr, nv))
-        vstep(IC<-1>{}, nb - 1, hc, true, cv, nr, nv);
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        v[q] = nv[q];
-        rr[q] = nr[q];
-      }
+      if (vscal(IC<-1>{}, nb - 1, hc, false, cv)) vscal(IC<-1>{}, nb - 1, hc, true, cv);
+      vvec(nb - 1, hc, cv, rr);
       Bv = cv.bv;
       keep_rotation(nb - 1, cv);
     }
@@ -997,16 +1008,12 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       constexpr int LV = (LY + 15) & 15;
       const double* hc = column(s - 2);
       VOut nvo;
-      double byn;
-      const bool rv = vstep(IC<LV>{}, s - 1, hc, false, nvo, nr, nv);
-      const bool ry = ystep_fast(IC<LY>{}, s, cv, rr, ny, byn);
-      if (__any(rv | ry | exact)) return false;
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        y[q] = ny[q];
-        v[q] = nv[q];
-        rr[q] = nr[q];
-      }
+      double byn, yq;
+      const bool rv = vscal(IC<LV>{}, s - 1, hc, false, nvo);
+      const bool ry = yscal(IC<LY>{}, s, cv, yq, byn);
+      if (__any(rv | ry | exact)) return false;   // nothing changed: the general step runs
+      yvec(s, yq, rr);
+      vvec(s - 1, hc, nvo, rr);   // R column of step s-1 replaces that of step s
       By = byn;
       Bv = nvo.bv;
       cv = nvo;
@@ -1017,21 +1024,15 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
     auto full_step = [&](int s) __attribute__((always_inline)) {
       const double* hc = column(s - 2);
       VOut nvo;
-      double byn;
-      if (vstep(IC<-1>{}, s - 1, hc, false, nvo, nr, nv))
-        vstep(IC<-1>{}, s - 1, hc, true, nvo, nr, nv);
-      if (ystep_fast(IC<-1>{}, s, cv, rr, ny, byn) || exact) {
+      double byn, yq;
+      if (vscal(IC<-1>{}, s - 1, hc, false, nvo)) vscal(IC<-1>{}, s - 1, hc, true, nvo);
+      if (yscal(IC<-1>{}, s, cv, yq, byn) || exact) {
         ystep_slow(s, cv, rr);
       } else {
-#pragma unroll
-        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        yvec(s, yq, rr);
         By = byn;
       }
-#pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        v[q] = nv[q];
-        rr[q] = nr[q];
-      }
+      vvec(s - 1, hc, nvo, rr);
       Bv = nvo.bv;
       cv = nvo;
       keep_rotation(s - 1, cv);
@@ -1078,12 +1079,11 @@ __global__ __launch_bounds__(256) void k_diag16(const ProbDev* __restrict__ prob
       --s;
     }
     {
-      double byn;
-      if (ystep_fast(IC<1>{}, 1, cv, rr, ny, byn) || exact) {
+      double byn, yq;
+      if (yscal(IC<1>{}, 1, cv, yq, byn) || exact) {
         ystep_slow(1, cv, rr);
       } else {
-#pragma unroll
-        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
+        yvec(1, yq, rr);
         By = byn;
       }
     }

@@ -9,6 +9,9 @@ The compute path is libhsrq.so (HIP kernels + C-ABI, ``include/hsrq.h`` and
   the drop-in symbol of the same name;
 * ``Context`` -- device-resident solves, batches, reusable plans (HIP-graph
   capture), on-device residuals;
+* ``Context.gaussian_hessenberg / constructed_hessenberg / grade`` -- test
+  problems generated in HBM (Philox Gaussian, prescribed spectrum, exact
+  power-of-two grading with a range check);
 * ``Server`` (``hsrq3in_amd.serve``) -- continuous-batching request loop.
 
 There is no fallback: if libhsrq.so is missing or fails to load, every call
@@ -27,6 +30,7 @@ LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
 
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2
+HSRQ_ERANGE = -8    # hsrq_grade_device: a graded entry would leave the FP64 range
 PREC_FP64 = 0
 PREC_FP32 = 1       # FP32 MFMA panel pass, FP64-residual check, FP64 re-solve of failures
 PREC_FP32_RAW = 2   # FP32 MFMA panel pass only
@@ -60,6 +64,7 @@ EXPORTS = [
     "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes", "hsrq_autotune",
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
     "hsrq_residuals_device",
+    "hsrq_gaussian_hessenberg_device", "hsrq_constructed_hessenberg_device", "hsrq_grade_device",
     "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
     "hsrq_server_get_stats", "hsrq_server_destroy",
 ]
@@ -109,6 +114,12 @@ def lib():
         L.hsrq_plan_destroy.argtypes = [_vp]
         L.hsrq_residuals_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                             _vp, _vp, ctypes.c_int, _dp]
+        L.hsrq_gaussian_hessenberg_device.argtypes = [_vp, ctypes.c_int, ctypes.c_uint64, _vp,
+                                                      ctypes.c_int]
+        L.hsrq_constructed_hessenberg_device.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                                         _vp, ctypes.c_int]
+        L.hsrq_grade_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _ip, ctypes.c_int,
+                                        _ip, _ip]
         _lib = L
     return _lib
 
@@ -285,6 +296,61 @@ class Context:
                                            wrt.data_ptr(), vrt.data_ptr(), vrt.shape[1],
                                            res.ctypes.data_as(_dp)), "hsrq_residuals_device")
         return res[:m]
+
+
+    # -- problem generation (column-major via transposed storage, like solve) --
+
+    def _drain(self):
+        import torch
+        torch.cuda.ExternalStream(self.stream_ptr()).synchronize()
+
+    def gaussian_hessenberg(self, n: int, seed: int, device: str = "cuda"):
+        """Tensor Ht (n, n), Ht[j, i] = H(i, j): N(0, 1) on and above the
+        subdiagonal (Philox4x32-10 + Box-Muller, hsrq_gaussian_hessenberg_device)."""
+        import torch
+        Ht = torch.empty((n, n), dtype=torch.float64, device=device)
+        torch.cuda.synchronize(Ht.device)
+        _check(lib().hsrq_gaussian_hessenberg_device(self._h, n, seed & (2**64 - 1),
+                                                     Ht.data_ptr(), n),
+               "hsrq_gaussian_hessenberg_device")
+        self._drain()
+        return Ht
+
+    def constructed_hessenberg(self, lam, nu, d, u, device: str = "cuda"):
+        """Tensor Ht of H = U B U^-1 (B = diag(lam) + subdiag(nu), U = diag(d)
+        + superdiag(u)): eigenvalues lam (hsrq_constructed_hessenberg_device)."""
+        import torch
+        lam, d = (np.ascontiguousarray(a, dtype=np.float64) for a in (lam, d))
+        n = lam.size
+        nu, u = (np.ascontiguousarray(a, dtype=np.float64) for a in (nu, u))
+        if d.size != n or nu.size != max(n - 1, 0) or u.size != max(n - 1, 0):
+            raise ValueError("lam, d need n entries and nu, u n-1")
+        Ht = torch.empty((n, n), dtype=torch.float64, device=device)
+        torch.cuda.synchronize(Ht.device)
+        # a one-entry array stands in for nu, u when n == 1
+        nu_, u_ = (a if a.size else np.zeros(1) for a in (nu, u))
+        _check(lib().hsrq_constructed_hessenberg_device(
+            self._h, n, lam.ctypes.data_as(_dp), nu_.ctypes.data_as(_dp),
+            d.ctypes.data_as(_dp), u_.ctypes.data_as(_dp), Ht.data_ptr(), n),
+            "hsrq_constructed_hessenberg_device")
+        return Ht
+
+    def grade(self, Ht, k, exact: bool = False):
+        """In place: H(i, j) *= 2^(k_i - k_j) on the Hessenberg part, after the
+        range check (hsrq_grade_device).  Returns (emax, emin), the extreme
+        binary exponents of the graded entries; raises HsrqError (HSRQ_ERANGE,
+        Ht untouched) on overflow or, if exact, on a result that rounds."""
+        import torch
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        n = Ht.shape[0]
+        if k.size != n:
+            raise ValueError("k needs n exponents")
+        emax, emin = ctypes.c_int(), ctypes.c_int()
+        torch.cuda.synchronize(Ht.device)
+        _check(lib().hsrq_grade_device(self._h, n, Ht.data_ptr(), Ht.shape[1],
+                                       k.ctypes.data_as(_ip), int(exact), ctypes.byref(emax),
+                                       ctypes.byref(emin)), "hsrq_grade_device")
+        return emax.value, emin.value
 
 
 class Plan:

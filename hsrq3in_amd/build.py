@@ -22,7 +22,8 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = ["hsrq_kernels.hip", "hsrq_sched.cpp", "hsrq_api.cpp", "hsrq_server.cpp"]
+SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_sched.cpp", "hsrq_api.cpp",
+           "hsrq_server.cpp"]
 
 
 def _run(cmd):

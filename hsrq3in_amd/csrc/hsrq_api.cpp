@@ -818,6 +818,7 @@ const char* hsrq_strerror(int code) {
     case HSRQ_ESELFTEST: return "device self-test failed (MFMA operand layout)";
     case HSRQ_EBUSY: return "capacity exceeded";
     case HSRQ_ENOTFOUND: return "unknown request";
+    case HSRQ_ERANGE: return "result outside the floating-point range";
     default: return "unknown error";
   }
 }

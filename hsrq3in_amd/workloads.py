@@ -73,6 +73,13 @@ def constructed_spectrum(lam: np.ndarray, seed: int, sub: float = 0.15,
     ``weak_nu`` (times the same random factor), so that eigenvalues closer
     together than nu (a cluster) keep well-conditioned eigenvectors.
     """
+    nu, d, u = constructed_parameters(lam, seed, sub, ratio, weak, weak_nu)
+    return constructed_from_parameters(lam, nu, d, u)
+
+
+def constructed_parameters(lam: np.ndarray, seed: int, sub: float = 0.15, ratio: float = 0.9,
+                           weak=None, weak_nu: float = 0.0):
+    """(nu, d, u) of ``constructed_spectrum`` (seeded)."""
     lam = np.asarray(lam, dtype=np.float64)
     n = lam.size
     rng = np.random.default_rng(seed)
@@ -86,6 +93,16 @@ def constructed_spectrum(lam: np.ndarray, seed: int, sub: float = 0.15,
                     nu[q] = weak_nu * f[q]
     d = rng.uniform(1.0, 2.0, size=n)
     u = rng.uniform(-ratio, ratio, size=max(n - 1, 0)) * d[1:]
+    return nu, d, u
+
+
+def constructed_from_parameters(lam, nu, d, u) -> np.ndarray:
+    """H = U B U^-1 for B = diag(lam) + subdiag(nu), U = diag(d) + superdiag(u).
+
+    The device build (Context.constructed_hessenberg) performs the same
+    operations in the same order and agrees bit for bit."""
+    lam = np.asarray(lam, dtype=np.float64)
+    n = lam.size
     # UB is tridiagonal: (UB)[i,i] = d_i lam_i + u_i nu_i, (UB)[i+1,i] = d_{i+1} nu_i,
     # (UB)[i,i+1] = u_i lam_{i+1}.
     x = np.zeros((n, n), dtype=np.float64, order="F")

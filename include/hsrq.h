@@ -31,7 +31,8 @@ enum {
     HSRQ_EUNSUPPORTED = -4,/* size outside the supported range            */
     HSRQ_ESELFTEST = -5,   /* device self-test failed                     */
     HSRQ_EBUSY = -6,       /* server queue full / capacity exceeded       */
-    HSRQ_ENOTFOUND = -7    /* unknown request id                          */
+    HSRQ_ENOTFOUND = -7,   /* unknown request id                          */
+    HSRQ_ERANGE = -8       /* a result would leave the FP64 range         */
 };
 
 /* Per-shift status bits (hsrq_*_status arrays). */
@@ -206,6 +207,34 @@ void hsrq_server_destroy(hsrq_server *srv);
  * per eigenvector, computed on the device (validator).  res: host, m. */
 int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
                           const double *dwr, const double *dvr, int ldvr, double *res);
+
+/* ---- test-problem generation on the device (the step before the solve) ----
+ * The reference builds its problem on the host (problemgenerator.c:305-356:
+ * random quasi-triangular T, Householder similarity, dgehrd; O(n^3)).  These
+ * build the workloads of this library's tests and benchmarks in HBM instead.
+ * n <= 65535; dH is column-major with leading dimension ldH >= n; the whole
+ * n x n matrix is written (zeros below the subdiagonal).  Enqueued on the
+ * context stream; the last two return after the stream has drained. */
+
+/* H(i,j) ~ N(0,1) for i <= j+1: Philox4x32-10 with key = seed and counter
+ * (j n + i) / 2, Box-Muller (cosine for even j n + i, sine for odd). */
+int hsrq_gaussian_hessenberg_device(hsrq_context *ctx, int n, uint64_t seed, double *dH,
+                                    int ldH);
+/* H = U B U^-1 with B = diag(lam) + subdiag(nu), U = diag(d) + superdiag(u):
+ * an unreduced upper Hessenberg matrix with eigenvalues lam.  lam, d: n
+ * entries; nu, u: n-1 (host).  d_i != 0. */
+int hsrq_constructed_hessenberg_device(hsrq_context *ctx, int n, const double *lam,
+                                       const double *nu, const double *d, const double *u,
+                                       double *dH, int ldH);
+/* Grading H <- D H D^-1, D = diag(2^k): H(i,j) *= 2^(k_i - k_j) for i <= j+1.
+ * k: host, n exponents, |k_i| <= 2^20.  A range check runs first: over the
+ * nonzero entries, emax / emin (may be NULL) receive the largest and smallest
+ * binary exponent E after scaling (|h| = f 2^E, 1/2 <= f < 1; INT_MIN /
+ * INT_MAX if H is zero).  HSRQ_ERANGE, with dH untouched, if an entry would
+ * overflow (E > 1024) or, with exact != 0, round (a result below the normal
+ * range that drops bits: the similarity would no longer be exact). */
+int hsrq_grade_device(hsrq_context *ctx, int n, double *dH, int ldH, const int *k, int exact,
+                      int *emax, int *emin);
 
 #ifdef __cplusplus
 }

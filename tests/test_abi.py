@@ -54,6 +54,11 @@ def test_invalid_arguments_rejected_without_device():
     assert lib.hsrq_context_set_tile(None, 64) == -1
     assert lib.hsrq_solve_device(None, 10, None, 10, 1, None, None, 10, 64, None) == -1
     assert lib.hsrq_plan_execute(None) == -1
+    assert lib.hsrq_gaussian_hessenberg_device(None, 10, 1, None, 10) == -1
+    assert lib.hsrq_grade_device(None, 10, None, 10, None, 0, None, None) == -1
+    assert lib.hsrq_constructed_hessenberg_device(None, 10, None, None, None, None, None,
+                                                  10) == -1
+    assert lib.hsrq_strerror(-8) == b"result outside the floating-point range"
 
 
 def test_bench_refuses_multi_gpu():

@@ -12,6 +12,8 @@ The compute path is libhsrq.so (HIP kernels + C-ABI, ``include/hsrq.h`` and
 * ``Context.gaussian_hessenberg / constructed_hessenberg / grade`` -- test
   problems generated in HBM (Philox Gaussian, prescribed spectrum, exact
   power-of-two grading with a range check);
+* ``Context.backtransform`` -- eigenvectors of A = Z H Z^T from those of H
+  (V = Z X, one FP64 GEMM);
 * ``Server`` (``hsrq3in_amd.serve``) -- continuous-batching request loop.
 
 There is no fallback: if libhsrq.so is missing or fails to load, every call
@@ -65,6 +67,7 @@ EXPORTS = [
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
     "hsrq_residuals_device",
     "hsrq_gaussian_hessenberg_device", "hsrq_constructed_hessenberg_device", "hsrq_grade_device",
+    "hsrq_backtransform_device",
     "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
     "hsrq_server_get_stats", "hsrq_server_destroy",
 ]
@@ -120,6 +123,8 @@ def lib():
                                                          _vp, ctypes.c_int]
         L.hsrq_grade_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _ip, ctypes.c_int,
                                         _ip, _ip]
+        L.hsrq_backtransform_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                                _vp, ctypes.c_int, _vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -351,6 +356,22 @@ class Context:
                                        k.ctypes.data_as(_ip), int(exact), ctypes.byref(emax),
                                        ctypes.byref(emin)), "hsrq_grade_device")
         return emax.value, emin.value
+
+
+    def backtransform(self, Zt, Xt, Vt=None):
+        """V = Z X on the context stream (hsrq_backtransform_device).  Zt (n, ldZ)
+        with Zt[j, i] = Z(i, j); Xt (m, ldX) holds the eigenvectors of H as rows
+        (like the solve's vrt); returns Vt (m, n) laid out the same way."""
+        import torch
+        n, m = Zt.shape[0], Xt.shape[0]
+        if Vt is None:
+            Vt = torch.empty((m, n), dtype=torch.float64, device=Xt.device)
+        torch.cuda.synchronize(Xt.device)
+        _check(lib().hsrq_backtransform_device(self._h, n, m, Zt.data_ptr(), Zt.shape[1],
+                                               Xt.data_ptr(), Xt.shape[1], Vt.data_ptr(),
+                                               Vt.shape[1]), "hsrq_backtransform_device")
+        self._drain()
+        return Vt
 
 
 class Plan:

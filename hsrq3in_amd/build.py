@@ -23,7 +23,9 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_sched.cpp", "hsrq_api.cpp",
-           "hsrq_server.cpp"]
+           "hsrq_server.cpp", "hsrq_blas.cpp"]
+# rocBLAS runs the back-transformation GEMM (hsrq_blas.cpp)
+LINK = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _run(cmd):
@@ -38,7 +40,7 @@ def _run(cmd):
 def build_library() -> str:
     out = os.path.join(HERE, "libhsrq.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", *SOURCES, "-o", tmp])
+    _run([HIPCC, *HIP_FLAGS, "-shared", *SOURCES, *LINK, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -764,6 +764,7 @@ int hsrq_context_create(hsrq_context** ctx, int device) {
 void hsrq_context_destroy(hsrq_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  release_blas(ctx);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

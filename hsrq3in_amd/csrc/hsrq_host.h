@@ -22,11 +22,14 @@ struct hsrq_context {
   int cus = 0;           // compute units of the device
   int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
   int lanes = 0;         // concurrent lanes of a plan (0: auto)
+  void* blas = nullptr;  // rocBLAS handle of the back-transformation, made on first use
 };
 
 namespace hsrq {
 
 hipError_t& last_hip_error();
+// Frees the context's rocBLAS handle (hsrq_blas.cpp).
+void release_blas(hsrq_context* ctx);
 
 #define HIP_TRY(expr)                                                 \
   do {                                                                \

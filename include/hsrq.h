@@ -236,6 +236,16 @@ int hsrq_constructed_hessenberg_device(hsrq_context *ctx, int n, const double *l
 int hsrq_grade_device(hsrq_context *ctx, int n, double *dH, int ldH, const int *k, int exact,
                       int *emax, int *emin);
 
+/* ---- back-transformation (the step after the solve) ----
+ * V = Z X: with A = Z H Z^T (Z orthogonal, e.g. the accumulated reflectors of
+ * a Hessenberg reduction) the columns of V are the eigenvectors of A for the
+ * eigenvectors X of H (unit 2-norm stays unit up to rounding).  Device
+ * column-major Z (n x n), X and V (n x m); V must not overlap Z or X.  One
+ * FP64 GEMM (rocBLAS) enqueued on the context stream.  The reference stops
+ * at the eigenvectors of H (inverse_iteration.h:19-22). */
+int hsrq_backtransform_device(hsrq_context *ctx, int n, int m, const double *dZ, int ldZ,
+                              const double *dX, int ldX, double *dV, int ldV);
+
 #ifdef __cplusplus
 }
 #endif

@@ -871,3 +871,234 @@ void oracle_residuals(int n, const double *H, int ldH, int m, const double *wr,
         free(r);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Complex shifts (the reference's unwired complex capability: its robust    */
+/* complex division robust.c:639-671 and solve_1x1_cmplx_system :677-767;    */
+/* its driver rejects complex eigenvalues, main.c:95-97).  Restatement of    */
+/* this library's complex-shift solve (k_csolve): one unblocked RQ sweep of  */
+/* H - sigma I with complex Givens rotations, the column-oriented back       */
+/* substitution fused into it, power-of-two protections, then x = U_{n-1}..  */
+/* U_1 z.  Written independently of the device code from the same equations. */
+/* ------------------------------------------------------------------------ */
+
+/* sqrt(a^2 + b^2 + c^2) with power-of-two scaling (no overflow). */
+static double norm3(double a, double b, double c)
+{
+    const double m = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+    if (m == 0.0 || !isfinite(m)) return m;
+    const int e = ilogb(m);
+    const double as = ldexp(a, -e), bs = ldexp(b, -e), cs = ldexp(c, -e);
+    return ldexp(sqrt((as * as + bs * bs) + cs * cs), e);
+}
+
+static int pow2_exponent(double alpha)
+{
+    return alpha >= 1.0 ? 0 : ilogb(alpha);
+}
+
+/* Unitary M = [[c, -conj(s)], [s, c]] on columns (h, v) = (st-1, st) with
+ * c h_st + s v_st = 0 for h_st = hsub real: c = |v|/d, s = -(hsub/d) conj(ph),
+ * ph = v/|v| (1 if v = 0), d = sqrt(|v|^2 + hsub^2); then R(st,st) = ph d. */
+static void crot(double vr, double vi, double hsub, double *c, double *sr, double *si,
+                 double *d, double *pr, double *pi)
+{
+    const double av = norm3(vr, vi, 0.0);
+    const double dd = norm3(vr, vi, hsub);
+    *d = dd;
+    if (dd == 0.0) {
+        *c = 1.0; *sr = 0.0; *si = 0.0; *pr = 1.0; *pi = 0.0;
+    } else if (av == 0.0) {
+        *c = 0.0; *pr = 1.0; *pi = 0.0;
+        *sr = -(hsub / dd); *si = 0.0;
+    } else {
+        *pr = vr / av; *pi = vi / av;
+        *c = av / dd;
+        const double t = hsub / dd;
+        *sr = -(t * *pr);
+        *si = t * *pi;
+    }
+}
+
+/* yq = y / (ph d), smin-protected (robust.c:639-671: a pivot below smin is
+ * replaced by the real smin), overflow-protected with a power of two 2^e
+ * (returned in *e1, applied to y).  Returns 1 if the pivot was raised. */
+static int cdiv_pivot(double smin, double d, double pr, double pi, double yr, double yi,
+                      double *qr, double *qi, int *e1)
+{
+    int info = 0;
+    if (d < smin) { d = smin; pr = 1.0; pi = 0.0; info = 1; }
+    double wr = yr * pr + yi * pi;
+    double wi = yi * pr - yr * pi;
+    *e1 = pow2_exponent(protect_real_division(fabs(wr) + fabs(wi), d));
+    if (*e1 != 0) { wr = ldexp(wr, *e1); wi = ldexp(wi, *e1); }
+    *qr = wr / d;
+    *qi = wi / d;
+    return info;
+}
+
+/* One complex shift; x (2n: re, im) unit 2-norm with its largest entry real
+ * positive.  Returns status bits (1 converged, 2 perturbed). */
+static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, double si0,
+                      double *xr, double *xi, double *C, double *SR, double *SI)
+{
+    const double eps = 0x1p-53;
+    const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));
+    const double asig = norm3(sr0, si0, 0.0);
+    const double smin = fmax(eps * asig, smlnum);
+    const double eps3 = hinf * 0x1p-53;
+    const double slack = 1.0 + 0x1p-40;
+    const double hm = hinf + asig;
+    double *vr = (double *)malloc(sizeof(double) * (size_t)n);
+    double *vi = (double *)malloc(sizeof(double) * (size_t)n);
+    double *yr = xr, *yi = xi;   /* y lives in x until the back-transform */
+    for (int i = 0; i < n; i++) {
+        vr[i] = H[i + (size_t)(n - 1) * ldH];
+        vi[i] = 0.0;
+        yr[i] = eps3;
+        yi[i] = 0.0;
+    }
+    vr[n - 1] -= sr0;
+    vi[n - 1] = -si0;
+    int E = 0, info = 0;
+    double bv = hm, by = eps3;
+    for (int st = n - 1; st >= 1; st--) {
+        double c, sr, si, d, pr, pi;
+        crot(vr[st], vi[st], H[st + (size_t)(st - 1) * ldH], &c, &sr, &si, &d, &pr, &pi);
+        C[st] = c; SR[st] = sr; SI[st] = si;
+        const double br = M_SQRT2 * norm3(bv, hm, 0.0) * slack;
+        double qr, qi;
+        int e1;
+        info |= cdiv_pivot(smin, d, pr, pi, yr[st], yi[st], &qr, &qi, &e1);
+        if (e1 != 0) {
+            for (int i = 0; i < n; i++) { yr[i] = ldexp(yr[i], e1); yi[i] = ldexp(yi[i], e1); }
+            by = ldexp(by, e1);
+            E += e1;
+        }
+        const double nq = fabs(qr) + fabs(qi);
+        double byn = (by + nq * br) * slack;
+        const double *hc = H + (size_t)(st - 1) * ldH;
+        if (!(byn <= g_omega)) {
+            /* exact bound a = max_i N(y_i) + N(yq) N(r_i) of the update, in the
+             * scaled form a 2^-kk (no overflow), then one power-of-two
+             * protection */
+            const int kk = nq > 0.0 ? ilogb(nq) : 0;
+            const double nqs = ldexp(nq, -kk);
+            double a = 0.0;
+            for (int i = 0; i < st; i++) {
+                const double hr = (i == st - 1) ? hc[i] - sr0 : hc[i];
+                const double hi = (i == st - 1) ? -si0 : 0.0;
+                const double rr = c * vr[i] - (sr * hr + si * hi);
+                const double ri = c * vi[i] - (sr * hi - si * hr);
+                a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(yr[i]) + fabs(yi[i]), -kk));
+            }
+            int e2 = 0;
+            if (a > ldexp(g_omega, -kk)) e2 = -(ilogb(a) + kk) - 1;
+            if (e2 != 0) {
+                for (int i = 0; i < n; i++) { yr[i] = ldexp(yr[i], e2); yi[i] = ldexp(yi[i], e2); }
+                qr = ldexp(qr, e2);
+                qi = ldexp(qi, e2);
+                E += e2;
+            }
+            byn = ldexp(a, kk + e2) * slack;
+        }
+        for (int i = 0; i < st; i++) {
+            const double hr = (i == st - 1) ? hc[i] - sr0 : hc[i];
+            const double hi = (i == st - 1) ? -si0 : 0.0;
+            const double rr = c * vr[i] - (sr * hr + si * hi);
+            const double ri = c * vi[i] - (sr * hi - si * hr);
+            const double nvr = c * hr + (sr * vr[i] - si * vi[i]);
+            const double nvi = c * hi + (sr * vi[i] + si * vr[i]);
+            yr[i] = yr[i] - (qr * rr - qi * ri);
+            yi[i] = yi[i] - (qr * ri + qi * rr);
+            vr[i] = nvr;
+            vi[i] = nvi;
+        }
+        yr[st] = qr;
+        yi[st] = qi;
+        by = byn;
+        bv = norm3(hm, bv, 0.0) * slack;
+    }
+    {   /* R(0,0) = v_0 */
+        const double d0 = norm3(vr[0], vi[0], 0.0);
+        const double pr = d0 > 0.0 ? vr[0] / d0 : 1.0, pi = d0 > 0.0 ? vi[0] / d0 : 0.0;
+        double qr, qi;
+        int e1;
+        info |= cdiv_pivot(smin, d0, pr, pi, yr[0], yi[0], &qr, &qi, &e1);
+        if (e1 != 0) {
+            for (int i = 1; i < n; i++) { yr[i] = ldexp(yr[i], e1); yi[i] = ldexp(yi[i], e1); }
+            E += e1;
+        }
+        yr[0] = qr;
+        yi[0] = qi;
+    }
+    free(vr);
+    free(vi);
+    /* x = U_{n-1} ... U_1 z, U_i = [[c, -conj(s)], [s, c]] on entries (i-1, i) */
+    double tr = yr[0], ti = yi[0];
+    for (int i = 1; i < n; i++) {
+        const double c = C[i], sr = SR[i], si = SI[i];
+        const double zr = yr[i], zi = yi[i];
+        xr[i - 1] = c * tr - (sr * zr + si * zi);
+        xi[i - 1] = c * ti - (sr * zi - si * zr);
+        const double ntr = (sr * tr - si * ti) + c * zr;
+        const double nti = (sr * ti + si * tr) + c * zi;
+        tr = ntr;
+        ti = nti;
+    }
+    xr[n - 1] = tr;
+    xi[n - 1] = ti;
+    /* ||x||_2, the first entry of largest modulus, normalisation */
+    double mx = 0.0;
+    for (int i = 0; i < n; i++) mx = fmax(mx, fmax(fabs(xr[i]), fabs(xi[i])));
+    double nrm = mx;
+    int p = 0;
+    if (mx > 0.0 && isfinite(mx)) {
+        const int e = ilogb(mx);
+        double sum = 0.0, best = -1.0;
+        for (int i = 0; i < n; i++) {
+            const double a = ldexp(xr[i], -e), b = ldexp(xi[i], -e);
+            const double q = a * a + b * b;
+            sum += q;
+            if (q > best) { best = q; p = i; }
+        }
+        nrm = ldexp(sqrt(sum), e);
+    }
+    const int conv = isfinite(nrm) && nrm > ldexp(0.1 / sqrt((double)n), E);
+    if (nrm > 0.0 && isfinite(nrm)) {
+        const double ap = norm3(xr[p], xi[p], 0.0);
+        const double pr = xr[p] / ap, pi = xi[p] / ap;
+        for (int i = 0; i < n; i++) {
+            const double a = xr[i], b = xi[i];
+            xr[i] = (a * pr + b * pi) / nrm;
+            xi[i] = (b * pr - a * pi) / nrm;
+        }
+    }
+    return (conv ? 1 : 0) | (info ? 2 : 0);
+}
+
+int oracle_complex_inverse_iteration(int n, const double *H, int ldH, int m, const double *wr,
+                                     const double *wi, double *vr, int ldvr, int *status,
+                                     int nthreads)
+{
+    if (n <= 0 || ldH < n || m < 0 || ldvr < n || !H || (m > 0 && (!wr || !wi || !vr)))
+        return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const double hinf = oracle_dlange_inf(n, n, H, ldH);
+#pragma omp parallel
+    {
+        double *w = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+#pragma omp for schedule(dynamic, 1)
+        for (int k = 0; k < m; k++) {
+            const int st = csolve_one(n, H, ldH, hinf, wr[k], wi[k], vr + (size_t)2 * k * ldvr,
+                                      vr + (size_t)(2 * k + 1) * ldvr, w, w + n, w + 2 * n);
+            if (status) status[k] = st;
+        }
+        free(w);
+    }
+    return 0;
+}

@@ -67,7 +67,7 @@ EXPORTS = [
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
     "hsrq_residuals_device",
     "hsrq_gaussian_hessenberg_device", "hsrq_constructed_hessenberg_device", "hsrq_grade_device",
-    "hsrq_backtransform_device",
+    "hsrq_backtransform_device", "hsrq_solve_complex_device",
     "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
     "hsrq_server_get_stats", "hsrq_server_destroy",
 ]
@@ -123,6 +123,8 @@ def lib():
                                                          _vp, ctypes.c_int]
         L.hsrq_grade_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _ip, ctypes.c_int,
                                         _ip, _ip]
+        L.hsrq_solve_complex_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
+                                                _vp, _vp, _vp, ctypes.c_int, _ip]
         L.hsrq_backtransform_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                                 _vp, ctypes.c_int, _vp, ctypes.c_int]
         _lib = L
@@ -302,6 +304,22 @@ class Context:
                                            res.ctypes.data_as(_dp)), "hsrq_residuals_device")
         return res[:m]
 
+
+    def solve_complex(self, Ht, wrt, wit, vrt=None):
+        """Complex shifts wr + i wi (device tensors, m entries) on Ht (as in
+        solve).  Returns (vrt, status): vrt (2m, n) with rows 2k / 2k+1 the real
+        / imaginary parts of x_k (hsrq_solve_complex_device)."""
+        import torch
+        n, m = Ht.shape[0], wrt.numel()
+        if vrt is None:
+            vrt = torch.zeros((2 * m, n), dtype=torch.float64, device=Ht.device)
+        st = np.zeros(max(m, 1), dtype=np.int32)
+        torch.cuda.synchronize(Ht.device)
+        _check(lib().hsrq_solve_complex_device(self._h, n, Ht.data_ptr(), Ht.shape[1], m,
+                                               wrt.data_ptr(), wit.data_ptr(), vrt.data_ptr(),
+                                               vrt.shape[1], st.ctypes.data_as(_ip)),
+               "hsrq_solve_complex_device")
+        return vrt, st[:m]
 
     # -- problem generation (column-major via transposed storage, like solve) --
 

@@ -22,8 +22,8 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_sched.cpp", "hsrq_api.cpp",
-           "hsrq_server.cpp", "hsrq_blas.cpp"]
+SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_cplx.hip", "hsrq_sched.cpp",
+           "hsrq_api.cpp", "hsrq_server.cpp", "hsrq_blas.cpp"]
 # rocBLAS runs the back-transformation GEMM (hsrq_blas.cpp)
 LINK = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 

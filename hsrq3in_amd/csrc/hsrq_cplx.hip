@@ -1,0 +1,552 @@
+// hsrq_cplx.hip -- inverse iteration with complex shifts sigma = a + i b on a
+// real upper Hessenberg H: the eigenvectors of complex-conjugate eigenpairs,
+// which the real-shift path (and the reference's driver, main.c:95-97) does
+// not produce.  The reference ships the robust complex primitives but no
+// caller (robust.c:639-671 robust_cmplx_div, :677-767 solve_1x1_cmplx_system).
+//
+// One workgroup of 512 threads per shift, the whole vector in registers:
+// thread t holds rows t + 512 q, q < RPT (n <= 512 RPT <= 8192).  The sweep
+// runs right to left over the columns of H - sigma I:
+//   * the running column v (column st after the rotations so far) and the
+//     next column h = H(:, st-1) (- sigma on the diagonal) are combined by the
+//     unitary M = [[c, -conj(s)], [s, c]] that zeroes row st of the new column
+//     st-1:  c = |v_st| / d, s = -(hsub / d) conj(ph), ph = v_st / |v_st|,
+//     d = sqrt(|v_st|^2 + hsub^2), so R(st,st) = ph d and
+//       r_i = c v_i - conj(s) h_i  (column st of R),
+//       v_i <- c h_i + s v_i       (the next running column);
+//   * the back substitution with R is fused into the sweep, column-oriented:
+//     y_st <- y_st / R(st,st) (pivot raised to smin if |R| < smin, as
+//     robust_cmplx_div; the division protected by a power of two), then
+//     y_i <- y_i - y_st r_i, with running bounds deciding whether a
+//     power-of-two protection of the update can be needed (then the exact
+//     bound is reduced over the workgroup);
+//   * x = U_{n-1} ... U_1 z from the stored (c, s): a sequential complex
+//     recurrence over LDS-staged chunks.
+// Every step is one barrier: the thread owning row st-1 publishes its new
+// (v, y) in LDS for the next rotation, which every thread computes alike.
+// Output: column 2k of vr the real part, 2k+1 the imaginary part of x_k,
+// ||x_k||_2 = 1 with its first entry of largest modulus real and positive;
+// status bit 1 converged (dlaein's growth test, as the real path), 2 a pivot
+// was raised to smin.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "hsrq_device.h"
+#include "hsrq_host.h"
+
+namespace hsrq {
+namespace {
+
+constexpr int kCThreads = 512;
+constexpr int kCWaves = kCThreads / 64;
+constexpr int kCMaxRpt = 16;
+constexpr int kCMaxN = kCThreads * kCMaxRpt;
+
+struct CArgs {
+  int n;
+  const double* H;
+  int64_t ldH;
+  const double* wr;
+  const double* wi;
+  double* vr;
+  int64_t ldvr;
+  int* status;
+  double* work;        // 3 n per shift: c, Re s, Im s of rotation st at index st
+  const double* rs;    // row sums ||H(i,:)||_1
+};
+
+// sqrt(a^2 + b^2 + c^2) with power-of-two scaling.
+__device__ __forceinline__ double norm3(double a, double b, double c) {
+  const double m = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+  if (m == 0.0 || !isfinite(m)) return m;
+  const int e = ilogb(m);
+  const double as = ldexp(a, -e), bs = ldexp(b, -e), cs = ldexp(c, -e);
+  return ldexp(sqrt((as * as + bs * bs) + cs * cs), e);
+}
+
+// Workgroup reductions (every thread of the workgroup calls them).
+__device__ __forceinline__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < kCWaves; ++w) r = fmax(r, red[w]);
+  return r;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < kCWaves; ++w) r = r + red[w];
+  return r;
+}
+
+__device__ __forceinline__ int block_min_int(int v, int* redi) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = redi[0];
+#pragma unroll
+  for (int w = 1; w < kCWaves; ++w) r = min(r, redi[w]);
+  return r;
+}
+
+// y / (ph d) with the pivot raised to smin below it and the division
+// protected by 2^e1 (applied to the quotient here, to the other entries by
+// the caller).
+__device__ __forceinline__ int cdiv_pivot(double smin, double d, double pr, double pi,
+                                          double yr, double yi, double& qr, double& qi,
+                                          int& e1) {
+  int info = 0;
+  if (d < smin) {
+    d = smin;
+    pr = 1.0;
+    pi = 0.0;
+    info = 1;
+  }
+  double wr = yr * pr + yi * pi;
+  double wi = yi * pr - yr * pi;
+  e1 = pow2_exp(protect_real_division(fabs(wr) + fabs(wi), d));
+  if (e1 != 0) {
+    wr = ldexp(wr, e1);
+    wi = ldexp(wi, e1);
+  }
+  qr = wr / d;
+  qi = wi / d;
+  return info;
+}
+
+template <int RPT>
+__global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
+  struct Rec {   // one pivot step: rotation, quotient, bounds, protections
+    double c, sr, si, qr, qi, by;
+    int e1, slow, info;
+  };
+  __shared__ Rec rec[2];                 // by step parity
+  __shared__ double red[kCWaves];
+  __shared__ int redi[kCWaves];
+  __shared__ double stg[5][kCThreads];   // back-transform chunk: Re z, Im z, c, Re s, Im s
+  __shared__ double xo[2][kCThreads + 1];
+
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = A.n;
+  const double* __restrict__ H = A.H;
+  const int64_t ldH = A.ldH;
+  const double sa = A.wr[k], sb = A.wi[k];
+  double* __restrict__ C = A.work + (int64_t)k * 3 * n;
+  double* __restrict__ SR = C + n;
+  double* __restrict__ SI = C + 2 * n;
+
+  double hinf = 0.0;
+  for (int i = tid; i < n; i += kCThreads) hinf = fmax(hinf, A.rs[i]);
+  hinf = block_max(hinf, red);
+
+  const double eps = 0x1p-53;
+  const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));
+  const double asig = norm3(sa, sb, 0.0);
+  const double smin = fmax(eps * asig, smlnum);
+  const double eps3 = hinf * 0x1p-53;
+  const double slack = 1.0 + 0x1p-40;
+  const double hm = hinf + asig;
+  // |h_i| <= hm, and a rotation keeps |v_i|^2 <= |h_i|^2 + |v_i|^2, so after j
+  // steps |v_i| <= hm sqrt(j + 1) <= hm sqrt(n); |r_i| <= sqrt(|v_i|^2 + |h_i|^2),
+  // and N(z) = |Re z| + |Im z| <= sqrt(2) |z|: br bounds N(r_i) in every step
+  const double br = M_SQRT2 * hm * sqrt((double)n + 1.0) * slack;
+
+  // H columns ride a ring of D register slots: the column of step st is
+  // loaded D steps before it is used (the step loop is unrolled by D so that
+  // every slot index is a compile-time constant).
+  constexpr int D = RPT <= 2 ? 8 : (RPT == 4 ? 2 : (RPT == 8 ? 4 : 1));
+  double vr[RPT], vi[RPT], yr[RPT], yi[RPT];
+  double ring[D][RPT], rsub[D];
+  auto load_col = [&](int st, double (&h)[RPT], double& hs) {
+    // column st-1 (rows 0 .. st-1 are used) and its subdiagonal entry
+    // H(st, st-1).  The loads are unconditional, from clamped in-bounds
+    // addresses, and their values are not touched until the step that uses
+    // them (which reads only rows <= st-1): no branch or select makes the
+    // compiler wait for them early.
+    const double* hcol = H + (int64_t)max(st - 1, 0) * ldH;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) h[q] = hcol[min(tid + kCThreads * q, n - 1)];
+    const int sc = min(max(st, 1), n - 1);
+    hs = H[n >= 2 ? sc + (int64_t)(sc - 1) * ldH : 0];
+  };
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int row = tid + kCThreads * q;
+    const bool in = row < n;
+    vr[q] = in ? H[row + (int64_t)(n - 1) * ldH] : 0.0;
+    vi[q] = 0.0;
+    yr[q] = in ? eps3 : 0.0;
+    yi[q] = 0.0;
+    if (row == n - 1) {
+      vr[q] -= sa;
+      vi[q] = -sb;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < D; ++u) load_col(n - 1 - u, ring[u], rsub[u]);
+  __syncthreads();
+
+  // The scalar chain of pivot step st -- rotation from (v_st, H(st, st-1)),
+  // division of y_st by R(st,st), bounds -- is computed once, by the
+  // wavefront that owns row st (right after it updated that row in step
+  // st+1), and published with the barrier; every thread then applies it.
+  const int wave = tid >> 6, lane = tid & 63;
+  auto chain = [&](int st, double psr, double psi, double ysr, double ysi, double hsub,
+                   double by) {
+    double c = 1.0, sr = 0.0, si = 0.0, pr = 1.0, pi = 0.0, av, d;
+    {   // |v| and d = sqrt(|v|^2 + hsub^2) under one power-of-two scaling
+      const double h = st >= 1 ? hsub : 0.0;
+      const double mm = fmax(fabs(psr), fmax(fabs(psi), fabs(h)));
+      if (mm == 0.0 || !isfinite(mm)) {
+        av = fmax(fabs(psr), fabs(psi));
+        d = mm;
+      } else {
+        const int e = ilogb(mm);
+        const double a = ldexp(psr, -e), b = ldexp(psi, -e), g = ldexp(h, -e);
+        const double s2v = a * a + b * b;
+        av = ldexp(sqrt(s2v), e);
+        d = ldexp(sqrt(s2v + g * g), e);
+      }
+    }
+    const bool tiny = d < smin;          // pivot raised to smin (robust_cmplx_div)
+    const double deff = tiny ? smin : d;
+    const double id = 1.0 / deff;
+    if (av > 0.0) {
+      const double ia = 1.0 / av;
+      pr = psr * ia;
+      pi = psi * ia;
+    }
+    if (st >= 1 && d > 0.0) {
+      const double idr = tiny ? 1.0 / d : id;
+      c = av * idr;
+      const double t = hsub * idr;
+      sr = -(t * pr);
+      si = t * pi;
+    }
+    // y_st / (ph d), the division protected by 2^e1
+    const double qpr = tiny ? 1.0 : pr, qpi = tiny ? 0.0 : pi;
+    double wr = ysr * qpr + ysi * qpi;
+    double wi = ysi * qpr - ysr * qpi;
+    const int e1 = pow2_exp(protect_real_division(fabs(wr) + fabs(wi), deff));
+    if (e1 != 0) {
+      wr = ldexp(wr, e1);
+      wi = ldexp(wi, e1);
+      by = ldexp(by, e1);
+    }
+    const double qr = wr * id, qi = wi * id;
+    const double byn = (by + (fabs(qr) + fabs(qi)) * br) * slack;
+    if (lane == 0) {
+      Rec& o = rec[st & 1];
+      o.c = c;
+      o.sr = sr;
+      o.si = si;
+      o.qr = qr;
+      o.qi = qi;
+      o.by = byn;
+      o.e1 = e1;
+      o.slow = !(byn <= kOmega);
+      o.info = tiny ? 1 : 0;
+      if (st >= 1) {
+        C[st] = c;
+        SR[st] = sr;
+        SI[st] = si;
+      }
+    }
+  };
+  // value of register slot q_own of the thread holding row `row`, in every lane
+  auto owner_value = [&](const double (&a)[RPT], int row) -> double {
+    const int qo = row / kCThreads;
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) v = (q == qo) ? a[q] : v;
+    return readlane(v, row & 63);
+  };
+
+  int E = 0, info = 0;
+  double by = eps3;
+  if (wave == ((n - 1) % kCThreads) >> 6)
+    chain(n - 1, owner_value(vr, n - 1), owner_value(vi, n - 1), owner_value(yr, n - 1),
+          owner_value(yi, n - 1), rsub[0], by);
+  __syncthreads();
+
+  auto step = [&](int st, const double (&hc)[RPT]) {
+    const Rec R = rec[st & 1];
+    const double c = R.c, sr = R.sr, si = R.si;
+    double qr = R.qr, qi = R.qi;
+    E += R.e1;
+    info |= R.info;
+    if (R.e1 != 0) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        yr[q] = ldexp(yr[q], R.e1);
+        yi[q] = ldexp(yi[q], R.e1);
+      }
+    }
+    by = R.by;
+    if (R.slow) {   // uniform: exact bound, scaled by 2^-kk
+      const double nq = fabs(qr) + fabs(qi);
+      const int kk = nq > 0.0 ? ilogb(nq) : 0;
+      const double nqs = ldexp(nq, -kk);
+      double a = 0.0;
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int row = tid + kCThreads * q;
+        if (row <= st - 1) {
+          const double hr = (row == st - 1) ? hc[q] - sa : hc[q];
+          const double hi = (row == st - 1) ? -sb : 0.0;
+          const double rr = c * vr[q] - (sr * hr + si * hi);
+          const double ri = c * vi[q] - (sr * hi - si * hr);
+          a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(yr[q]) + fabs(yi[q]), -kk));
+        }
+      }
+      a = block_max(a, red);
+      int e2 = 0;
+      if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
+      if (e2 != 0) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          yr[q] = ldexp(yr[q], e2);
+          yi[q] = ldexp(yi[q], e2);
+        }
+        qr = ldexp(qr, e2);
+        qi = ldexp(qi, e2);
+        E += e2;
+      }
+      by = ldexp(a, kk + e2) * slack;
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int row = tid + kCThreads * q;
+      if (row <= st - 1) {
+        const double hr = (row == st - 1) ? hc[q] - sa : hc[q];
+        const double hi = (row == st - 1) ? -sb : 0.0;
+        const double rr = fma(c, vr[q], -fma(sr, hr, si * hi));
+        const double ri = fma(c, vi[q], fma(si, hr, -(sr * hi)));
+        const double nvr = fma(c, hr, fma(sr, vr[q], -(si * vi[q])));
+        const double nvi = fma(c, hi, fma(sr, vi[q], si * vr[q]));
+        yr[q] = fma(-qr, rr, fma(qi, ri, yr[q]));
+        yi[q] = fma(-qr, ri, fma(-qi, rr, yi[q]));
+        vr[q] = nvr;
+        vi[q] = nvi;
+      } else if (row == st) {
+        yr[q] = qr;
+        yi[q] = qi;
+      }
+    }
+  };
+  for (int st0 = n - 1; st0 >= 1; st0 -= D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int st = st0 - u;
+      if (st >= 1) {
+        step(st, ring[u]);
+        load_col(st - D, ring[u], rsub[u]);   // the column of step st - D
+        if (wave == ((st - 1) % kCThreads) >> 6)
+          chain(st - 1, owner_value(vr, st - 1), owner_value(vi, st - 1),
+                owner_value(yr, st - 1), owner_value(yi, st - 1), rsub[(u + 1) % D], by);
+        __syncthreads();
+      }
+    }
+  }
+  {   // apply step 0: y_0 = y_0 / R(0,0)
+    const Rec R = rec[0];
+    E += R.e1;
+    info |= R.info;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int row = tid + kCThreads * q;
+      if (row == 0) {
+        yr[q] = R.qr;
+        yi[q] = R.qi;
+      } else if (R.e1 != 0) {
+        yr[q] = ldexp(yr[q], R.e1);
+        yi[q] = ldexp(yi[q], R.e1);
+      }
+    }
+  }
+  // the rotations written by thread 0 are read by all threads below
+  __threadfence_block();
+  __syncthreads();
+
+  // x = U_{n-1} ... U_1 z, chunk q = rows 512 q .. 512 q + 511 (slot q of
+  // every thread); xo[j] receives x of row 512 q + j - 1.
+  double tr = 0.0, ti = 0.0;   // the entering value (thread 0)
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int g0 = kCThreads * q;
+    if (g0 < n) {
+      const int row = g0 + tid;
+      if (row < n) {
+        stg[0][tid] = yr[q];
+        stg[1][tid] = yi[q];
+        stg[2][tid] = row >= 1 ? C[row] : 1.0;
+        stg[3][tid] = row >= 1 ? SR[row] : 0.0;
+        stg[4][tid] = row >= 1 ? SI[row] : 0.0;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const int cnt = min(kCThreads, n - g0);
+        for (int j = 0; j < cnt; ++j) {
+          const double zr = stg[0][j], zi = stg[1][j];
+          if (g0 + j == 0) {
+            tr = zr;
+            ti = zi;
+          } else {
+            const double cc = stg[2][j], s_r = stg[3][j], s_i = stg[4][j];
+            xo[0][j] = cc * tr - (s_r * zr + s_i * zi);
+            xo[1][j] = cc * ti - (s_r * zi - s_i * zr);
+            const double ntr = (s_r * tr - s_i * ti) + cc * zr;
+            const double nti = (s_r * ti + s_i * tr) + cc * zi;
+            tr = ntr;
+            ti = nti;
+          }
+        }
+      }
+      __syncthreads();
+      if (tid + 1 < kCThreads && row + 1 < n) {   // produced at element tid + 1
+        yr[q] = xo[0][tid + 1];
+        yi[q] = xo[1][tid + 1];
+      }
+      if (q > 0 && tid == kCThreads - 1) {        // last row of the previous chunk
+        yr[q > 0 ? q - 1 : 0] = xo[0][0];
+        yi[q > 0 ? q - 1 : 0] = xo[1][0];
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    xo[0][0] = tr;
+    xo[1][0] = ti;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    if (tid + kCThreads * q == n - 1) {
+      yr[q] = xo[0][0];
+      yi[q] = xo[1][0];
+    }
+
+  // ||x||_2, the first entry of largest modulus, normalisation
+  double mx = 0.0;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    if (tid + kCThreads * q < n) mx = fmax(mx, fmax(fabs(yr[q]), fabs(yi[q])));
+  mx = block_max(mx, red);
+  double nrm = mx;
+  int p = 0;
+  if (mx > 0.0 && isfinite(mx)) {
+    const int e = ilogb(mx);
+    double sum = 0.0, best = -1.0;
+    int bi = INT_MAX;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int row = tid + kCThreads * q;
+      if (row < n) {
+        const double a = ldexp(yr[q], -e), b = ldexp(yi[q], -e);
+        const double t = a * a + b * b;
+        sum += t;
+        if (t > best) {
+          best = t;
+          bi = row;
+        }
+      }
+    }
+    sum = block_sum(sum, red);
+    const double bmax = block_max(best, red);
+    p = block_min_int(best == bmax ? bi : INT_MAX, redi);
+    nrm = ldexp(sqrt(sum), e);
+  }
+  const bool conv = isfinite(nrm) && nrm > ldexp(0.1 / sqrt((double)n), E);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RPT; ++q)
+    if (tid + kCThreads * q == p) {
+      xo[0][0] = yr[q];
+      xo[1][0] = yi[q];
+    }
+  __syncthreads();
+  double pr = 1.0, pi = 0.0;
+  const bool scale = nrm > 0.0 && isfinite(nrm);
+  if (scale) {
+    const double ap = norm3(xo[0][0], xo[1][0], 0.0);
+    pr = xo[0][0] / ap;
+    pi = xo[1][0] / ap;
+  }
+  double* outr = A.vr + (int64_t)(2 * k) * A.ldvr;
+  double* outi = A.vr + (int64_t)(2 * k + 1) * A.ldvr;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int row = tid + kCThreads * q;
+    if (row < n) {
+      const double a = yr[q], b = yi[q];
+      outr[row] = scale ? (a * pr + b * pi) / nrm : a;
+      outi[row] = scale ? (b * pr - a * pi) / nrm : b;
+    }
+  }
+  if (tid == 0) A.status[k] = (conv ? HSRQ_CONVERGED : 0) | (info ? HSRQ_PERTURBED : 0);
+}
+
+template <int RPT>
+hipError_t launch_csolve(const CArgs& a, int m, hipStream_t st) {
+  hipLaunchKernelGGL(k_csolve<RPT>, dim3(m), dim3(kCThreads), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+}  // namespace hsrq
+
+using namespace hsrq;
+
+extern "C" int hsrq_solve_complex_device(hsrq_context* ctx, int n, const double* dH, int ldH,
+                                         int m, const double* dwr, const double* dwi,
+                                         double* dvr, int ldvr, int* status) {
+  if (!ctx || n <= 0 || m < 0 || !dH || ldH < n || ldvr < n ||
+      (m > 0 && (!dwr || !dwi || !dvr)))
+    return HSRQ_EINVAL;
+  if (n > kCMaxN) return HSRQ_EUNSUPPORTED;
+  if (m == 0) return HSRQ_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  // rotations (3 n per shift), row sums (n), their partial sums (64 n), status (m)
+  const size_t nd = 3 * (size_t)n * m + 65 * (size_t)n;
+  char* buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, nd * sizeof(double) + (size_t)m * sizeof(int)));
+  double* work = reinterpret_cast<double*>(buf);
+  double* rs = work + 3 * (size_t)n * m;
+  double* part = rs + n;
+  int* dst = reinterpret_cast<int*>(buf + nd * sizeof(double));
+  CArgs a{n, dH, ldH, dwr, dwi, dvr, ldvr, dst, work, rs};
+  hipError_t e = launch_rowsums(n, dH, ldH, part, rs, ctx->stream);
+  const int rpt = (n + kCThreads - 1) / kCThreads;
+  if (e == hipSuccess) {
+    if (rpt <= 1)
+      e = launch_csolve<1>(a, m, ctx->stream);
+    else if (rpt <= 2)
+      e = launch_csolve<2>(a, m, ctx->stream);
+    else if (rpt <= 4)
+      e = launch_csolve<4>(a, m, ctx->stream);
+    else if (rpt <= 8)
+      e = launch_csolve<8>(a, m, ctx->stream);
+    else
+      e = launch_csolve<16>(a, m, ctx->stream);
+  }
+  if (e == hipSuccess && status)
+    e = hipMemcpyAsync(status, dst, (size_t)m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = es;
+  (void)hipFree(buf);
+  HIP_TRY(e);
+  return HSRQ_OK;
+}

@@ -208,6 +208,21 @@ void hsrq_server_destroy(hsrq_server *srv);
 int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
                           const double *dwr, const double *dvr, int ldvr, double *res);
 
+/* ---- complex shifts: eigenvectors of complex-conjugate eigenpairs ----
+ * sigma_k = wr[k] + i wi[k] (device arrays, m entries).  One inverse-iteration
+ * solve (H - sigma_k I) x = eps3 (1, ..., 1) per shift with complex Givens
+ * rotations and power-of-two protections; no restarts.  dvr: n x 2m device
+ * column-major, column 2k the real and 2k+1 the imaginary part of x_k
+ * (LAPACK's layout for a complex pair), ||x_k||_2 = 1, its first entry of
+ * largest modulus real and positive.  status (host, m, may be NULL):
+ * HSRQ_CONVERGED (growth test of dlaein), HSRQ_PERTURBED.  n <= 8192
+ * (HSRQ_EUNSUPPORTED above).  The reference has the robust complex
+ * primitives (robust.c:639-767) but no complex-shift solve; its driver
+ * rejects complex eigenvalues (main.c:95-97). */
+int hsrq_solve_complex_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
+                              const double *dwr, const double *dwi, double *dvr, int ldvr,
+                              int *status);
+
 /* ---- test-problem generation on the device (the step before the solve) ----
  * The reference builds its problem on the host (problemgenerator.c:305-356:
  * random quasi-triangular T, Householder similarity, dgehrd; O(n^3)).  These

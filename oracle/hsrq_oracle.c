@@ -897,44 +897,60 @@ static int pow2_exponent(double alpha)
     return alpha >= 1.0 ? 0 : ilogb(alpha);
 }
 
-/* Unitary M = [[c, -conj(s)], [s, c]] on columns (h, v) = (st-1, st) with
+/* Pivot step st (st = 0: the last pivot R(0,0) = v_0).  Unitary
+ * M = [[c, -conj(s)], [s, c]] on columns (h, v) = (st-1, st) with
  * c h_st + s v_st = 0 for h_st = hsub real: c = |v|/d, s = -(hsub/d) conj(ph),
- * ph = v/|v| (1 if v = 0), d = sqrt(|v|^2 + hsub^2); then R(st,st) = ph d. */
-static void crot(double vr, double vi, double hsub, double *c, double *sr, double *si,
-                 double *d, double *pr, double *pi)
-{
-    const double av = norm3(vr, vi, 0.0);
-    const double dd = norm3(vr, vi, hsub);
-    *d = dd;
-    if (dd == 0.0) {
-        *c = 1.0; *sr = 0.0; *si = 0.0; *pr = 1.0; *pi = 0.0;
-    } else if (av == 0.0) {
-        *c = 0.0; *pr = 1.0; *pi = 0.0;
-        *sr = -(hsub / dd); *si = 0.0;
-    } else {
-        *pr = vr / av; *pi = vi / av;
-        *c = av / dd;
-        const double t = hsub / dd;
-        *sr = -(t * *pr);
-        *si = t * *pi;
-    }
-}
+ * ph = v/|v| (1 if v = 0), d = sqrt(|v|^2 + hsub^2), R(st,st) = ph d; then
+ * q = y_st / R(st,st) with the pivot raised to smin below it (robust.c:639-671
+ * replaces a tiny complex pivot by the real smin) and the division protected
+ * by 2^e1 (robust.c:144-166).  |v| and d share one power-of-two scaling; the
+ * divisions by |v| and d are reciprocals taken once. */
+typedef struct { double c, sr, si, qr, qi; int e1, info; } cpivot_t;
 
-/* yq = y / (ph d), smin-protected (robust.c:639-671: a pivot below smin is
- * replaced by the real smin), overflow-protected with a power of two 2^e
- * (returned in *e1, applied to y).  Returns 1 if the pivot was raised. */
-static int cdiv_pivot(double smin, double d, double pr, double pi, double yr, double yi,
-                      double *qr, double *qi, int *e1)
+static cpivot_t cpivot(int st, double vr, double vi, double hsub, double yr, double yi,
+                       double smin)
 {
-    int info = 0;
-    if (d < smin) { d = smin; pr = 1.0; pi = 0.0; info = 1; }
-    double wr = yr * pr + yi * pi;
-    double wi = yi * pr - yr * pi;
-    *e1 = pow2_exponent(protect_real_division(fabs(wr) + fabs(wi), d));
-    if (*e1 != 0) { wr = ldexp(wr, *e1); wi = ldexp(wi, *e1); }
-    *qr = wr / d;
-    *qi = wi / d;
-    return info;
+    cpivot_t o = {1.0, 0.0, 0.0, 0.0, 0.0, 0, 0};
+    double pr = 1.0, pi = 0.0, av, d;
+    const double h = st >= 1 ? hsub : 0.0;
+    const double mm = fmax(fabs(vr), fmax(fabs(vi), fabs(h)));
+    if (mm == 0.0 || !isfinite(mm)) {
+        av = fmax(fabs(vr), fabs(vi));
+        d = mm;
+    } else {
+        const int e = ilogb(mm);
+        const double a = ldexp(vr, -e), b = ldexp(vi, -e), g = ldexp(h, -e);
+        const double s2v = a * a + b * b;
+        av = ldexp(sqrt(s2v), e);
+        d = ldexp(sqrt(s2v + g * g), e);
+    }
+    const int tiny = d < smin;
+    const double deff = tiny ? smin : d;
+    const double id = 1.0 / deff;
+    if (av > 0.0) {
+        const double ia = 1.0 / av;
+        pr = vr * ia;
+        pi = vi * ia;
+    }
+    if (st >= 1 && d > 0.0) {
+        const double idr = tiny ? 1.0 / d : id;
+        o.c = av * idr;
+        const double t = hsub * idr;
+        o.sr = -(t * pr);
+        o.si = t * pi;
+    }
+    const double qpr = tiny ? 1.0 : pr, qpi = tiny ? 0.0 : pi;
+    double wr = yr * qpr + yi * qpi;
+    double wi = yi * qpr - yr * qpi;
+    o.e1 = pow2_exponent(protect_real_division(fabs(wr) + fabs(wi), deff));
+    if (o.e1 != 0) {
+        wr = ldexp(wr, o.e1);
+        wi = ldexp(wi, o.e1);
+    }
+    o.qr = wr * id;
+    o.qi = wi * id;
+    o.info = tiny;
+    return o;
 }
 
 /* One complex shift; x (2n: re, im) unit 2-norm with its largest entry real
@@ -949,6 +965,9 @@ static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, 
     const double eps3 = hinf * 0x1p-53;
     const double slack = 1.0 + 0x1p-40;
     const double hm = hinf + asig;
+    /* |v_i| <= hm sqrt(j + 1) after j rotations, |r_i| <= sqrt(|v_i|^2 + |h_i|^2),
+     * |Re z| + |Im z| <= sqrt(2) |z|: a bound of |Re r_i| + |Im r_i| in every step */
+    const double br = M_SQRT2 * hm * sqrt((double)n + 1.0) * slack;
     double *vr = (double *)malloc(sizeof(double) * (size_t)n);
     double *vi = (double *)malloc(sizeof(double) * (size_t)n);
     double *yr = xr, *yi = xi;   /* y lives in x until the back-transform */
@@ -961,25 +980,24 @@ static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, 
     vr[n - 1] -= sr0;
     vi[n - 1] = -si0;
     int E = 0, info = 0;
-    double bv = hm, by = eps3;
+    double by = eps3;
     for (int st = n - 1; st >= 1; st--) {
-        double c, sr, si, d, pr, pi;
-        crot(vr[st], vi[st], H[st + (size_t)(st - 1) * ldH], &c, &sr, &si, &d, &pr, &pi);
+        const cpivot_t P = cpivot(st, vr[st], vi[st], H[st + (size_t)(st - 1) * ldH], yr[st],
+                                  yi[st], smin);
+        const double c = P.c, sr = P.sr, si = P.si;
+        double qr = P.qr, qi = P.qi;
         C[st] = c; SR[st] = sr; SI[st] = si;
-        const double br = M_SQRT2 * norm3(bv, hm, 0.0) * slack;
-        double qr, qi;
-        int e1;
-        info |= cdiv_pivot(smin, d, pr, pi, yr[st], yi[st], &qr, &qi, &e1);
-        if (e1 != 0) {
-            for (int i = 0; i < n; i++) { yr[i] = ldexp(yr[i], e1); yi[i] = ldexp(yi[i], e1); }
-            by = ldexp(by, e1);
-            E += e1;
+        info |= P.info;
+        if (P.e1 != 0) {
+            for (int i = 0; i < n; i++) { yr[i] = ldexp(yr[i], P.e1); yi[i] = ldexp(yi[i], P.e1); }
+            by = ldexp(by, P.e1);
+            E += P.e1;
         }
         const double nq = fabs(qr) + fabs(qi);
         double byn = (by + nq * br) * slack;
         const double *hc = H + (size_t)(st - 1) * ldH;
         if (!(byn <= g_omega)) {
-            /* exact bound a = max_i N(y_i) + N(yq) N(r_i) of the update, in the
+            /* exact bound a = max_i N(y_i) + N(q) N(r_i) of the update, in the
              * scaled form a 2^-kk (no overflow), then one power-of-two
              * protection */
             const int kk = nq > 0.0 ? ilogb(nq) : 0;
@@ -1002,35 +1020,33 @@ static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, 
             }
             byn = ldexp(a, kk + e2) * slack;
         }
+        /* r_i = c v_i - conj(s) h_i, v_i <- c h_i + s v_i, y_i <- y_i - q r_i
+         * (fused multiply-adds) */
         for (int i = 0; i < st; i++) {
             const double hr = (i == st - 1) ? hc[i] - sr0 : hc[i];
             const double hi = (i == st - 1) ? -si0 : 0.0;
-            const double rr = c * vr[i] - (sr * hr + si * hi);
-            const double ri = c * vi[i] - (sr * hi - si * hr);
-            const double nvr = c * hr + (sr * vr[i] - si * vi[i]);
-            const double nvi = c * hi + (sr * vi[i] + si * vr[i]);
-            yr[i] = yr[i] - (qr * rr - qi * ri);
-            yi[i] = yi[i] - (qr * ri + qi * rr);
+            const double rr = fma(c, vr[i], -fma(sr, hr, si * hi));
+            const double ri = fma(c, vi[i], fma(si, hr, -(sr * hi)));
+            const double nvr = fma(c, hr, fma(sr, vr[i], -(si * vi[i])));
+            const double nvi = fma(c, hi, fma(sr, vi[i], si * vr[i]));
+            yr[i] = fma(-qr, rr, fma(qi, ri, yr[i]));
+            yi[i] = fma(-qr, ri, fma(-qi, rr, yi[i]));
             vr[i] = nvr;
             vi[i] = nvi;
         }
         yr[st] = qr;
         yi[st] = qi;
         by = byn;
-        bv = norm3(hm, bv, 0.0) * slack;
     }
     {   /* R(0,0) = v_0 */
-        const double d0 = norm3(vr[0], vi[0], 0.0);
-        const double pr = d0 > 0.0 ? vr[0] / d0 : 1.0, pi = d0 > 0.0 ? vi[0] / d0 : 0.0;
-        double qr, qi;
-        int e1;
-        info |= cdiv_pivot(smin, d0, pr, pi, yr[0], yi[0], &qr, &qi, &e1);
-        if (e1 != 0) {
-            for (int i = 1; i < n; i++) { yr[i] = ldexp(yr[i], e1); yi[i] = ldexp(yi[i], e1); }
-            E += e1;
+        const cpivot_t P = cpivot(0, vr[0], vi[0], 0.0, yr[0], yi[0], smin);
+        info |= P.info;
+        if (P.e1 != 0) {
+            for (int i = 1; i < n; i++) { yr[i] = ldexp(yr[i], P.e1); yi[i] = ldexp(yi[i], P.e1); }
+            E += P.e1;
         }
-        yr[0] = qr;
-        yi[0] = qi;
+        yr[0] = P.qr;
+        yi[0] = P.qi;
     }
     free(vr);
     free(vi);

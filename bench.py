@@ -155,6 +155,64 @@ def batched(ctx, steps, warmup, tile, plain_sample):
             "kernels": {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}}
 
 
+def around_path(ctx, reps=3):
+    """The steps either side of the path, at the plain workload's order n=8000:
+    the device Gaussian Hessenberg generator, the complex-shift solve (m=256
+    shifts drawn in the spectral disc |z| < sqrt(n), one workgroup per shift),
+    and the back-transformation V = Z X of m=800 eigenvectors (FP64 GEMM).
+    Each is timed with HIP events on the context stream, inputs in HBM."""
+    import torch
+    from hsrq3in_amd import lib
+    n = 8000
+    L = lib()
+    st = torch.cuda.ExternalStream(ctx.stream_ptr())
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    print("[bench] around the path", file=sys.stderr, flush=True)
+    Ht = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    gen_ms = timed(lambda: L.hsrq_gaussian_hessenberg_device(ctx.handle, n, 11, Ht.data_ptr(), n))
+    mc = 256
+    rng = np.random.default_rng(17)
+    rad = np.sqrt(n) * np.sqrt(rng.uniform(0.0, 1.0, mc))
+    ang = rng.uniform(0.05, np.pi - 0.05, mc)
+    wr = torch.from_numpy(rad * np.cos(ang)).cuda()
+    wi = torch.from_numpy(rad * np.sin(ang)).cuda()
+    vc = torch.zeros((2 * mc, n), dtype=torch.float64, device="cuda")
+    ctx.solve_complex(Ht, wr, wi, vc)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _, cst = ctx.solve_complex(Ht, wr, wi, vc)
+    cplx_ms = 1e3 * (time.perf_counter() - t0) / reps
+    del vc
+    m = 800
+    X = torch.randn((m, n), dtype=torch.float64, device="cuda")
+    V = torch.empty_like(X)
+    bt_ms = timed(lambda: L.hsrq_backtransform_device(ctx.handle, n, m, Ht.data_ptr(), n,
+                                                      X.data_ptr(), n, V.data_ptr(), n))
+    return {
+        "gaussian_generator": {"n": n, "ms": round(gen_ms, 3),
+                               "written_GBs": round(8 * n * n / gen_ms / 1e6, 1)},
+        "complex_shifts": {"n": n, "m": mc, "ms": round(cplx_ms, 2),
+                           "eigvecs_per_s": round(mc / (cplx_ms / 1e3), 1),
+                           "converged": int(np.sum((cst & 1) != 0)),
+                           "timing": "wall clock of hsrq_solve_complex_device (synchronous)"},
+        "backtransform": {"n": n, "m": m, "ms": round(bt_ms, 3),
+                          "TFLOPs": round(2.0 * n * n * m / bt_ms / 1e9, 1),
+                          "frac_fp64_mfma_peak": round(2.0 * n * n * m / bt_ms / 1e9
+                                                       / PEAK_FP64_TFLOPS, 3)},
+    }
+
+
 def plain_variant(ctx, kind, steps, tile):
     """The plain workload's two variants at n=8000, m=800 (same timing as the
     headline line): 'graded' -- the same Gaussian H as D H D^-1, D = diag(2^k),
@@ -362,6 +420,8 @@ def main():
     ap.add_argument("--batch-packing", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the generator / complex-shift / back-transformation timings")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the n=20000, m=2000 reduced-precision comparison")
     ap.add_argument("--no-serve", action="store_true")
@@ -468,6 +528,8 @@ def main():
     if (n, m) == (8000, 800) and not args.no_variants:
         out["variants"] = {k: plain_variant(ctx, k, max(1, args.steps // 2), tile)
                            for k in ("graded", "cluster")}
+    if not args.no_extras:
+        out["around_the_path"] = around_path(ctx)
     if not args.no_batch:
         ctx.set_packing(args.batch_packing)
         out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 32)

@@ -126,8 +126,11 @@ __device__ __forceinline__ int cdiv_pivot(double smin, double d, double pr, doub
   return info;
 }
 
-template <int RPT>
+// YL: the right-hand side y lives in LDS instead of registers (the layout of
+// 16 rows per thread, whose registers would otherwise spill).
+template <int RPT, bool YL>
 __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
+  __shared__ double ylr[YL ? kCMaxN : 1], yli[YL ? kCMaxN : 1];
   struct Rec {   // one pivot step: rotation, quotient, bounds, protections
     double c, sr, si, qr, qi, by;
     int e1, slow, info;
@@ -168,7 +171,9 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   // loaded D steps before it is used (the step loop is unrolled by D so that
   // every slot index is a compile-time constant).
   constexpr int D = RPT <= 2 ? 8 : (RPT == 4 ? 2 : (RPT == 8 ? 4 : 1));
-  double vr[RPT], vi[RPT], yr[RPT], yi[RPT];
+  double vr[RPT], vi[RPT], yrr[YL ? 1 : RPT], yir[YL ? 1 : RPT];
+#define YR(q) (*(YL ? &ylr[tid + kCThreads * (q)] : &yrr[YL ? 0 : (q)]))
+#define YI(q) (*(YL ? &yli[tid + kCThreads * (q)] : &yir[YL ? 0 : (q)]))
   double ring[D][RPT], rsub[D];
   auto load_col = [&](int st, double (&h)[RPT], double& hs) {
     // column st-1 (rows 0 .. st-1 are used) and its subdiagonal entry
@@ -188,8 +193,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     const bool in = row < n;
     vr[q] = in ? H[row + (int64_t)(n - 1) * ldH] : 0.0;
     vi[q] = 0.0;
-    yr[q] = in ? eps3 : 0.0;
-    yi[q] = 0.0;
+    YR(q) = in ? eps3 : 0.0;
+    YI(q) = 0.0;
     if (row == n - 1) {
       vr[q] -= sa;
       vi[q] = -sb;
@@ -275,11 +280,20 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     return readlane(v, row & 63);
   };
 
+  // y of row `row` in every lane: from LDS, or from the owner's register
+  auto owner_y = [&](const double (&a)[YL ? 1 : RPT], const double* l, int row) -> double {
+    if constexpr (YL) {
+      return l[row];
+    } else {
+      return owner_value(a, row);
+    }
+  };
+
   int E = 0, info = 0;
   double by = eps3;
   if (wave == ((n - 1) % kCThreads) >> 6)
-    chain(n - 1, owner_value(vr, n - 1), owner_value(vi, n - 1), owner_value(yr, n - 1),
-          owner_value(yi, n - 1), rsub[0], by);
+    chain(n - 1, owner_value(vr, n - 1), owner_value(vi, n - 1), owner_y(yrr, ylr, n - 1),
+          owner_y(yir, yli, n - 1), rsub[0], by);
   __syncthreads();
 
   auto step = [&](int st, const double (&hc)[RPT]) {
@@ -291,8 +305,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     if (R.e1 != 0) {
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        yr[q] = ldexp(yr[q], R.e1);
-        yi[q] = ldexp(yi[q], R.e1);
+        YR(q) = ldexp(YR(q), R.e1);
+        YI(q) = ldexp(YI(q), R.e1);
       }
     }
     by = R.by;
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
           const double hi = (row == st - 1) ? -sb : 0.0;
           const double rr = c * vr[q] - (sr * hr + si * hi);
           const double ri = c * vi[q] - (sr * hi - si * hr);
-          a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(yr[q]) + fabs(yi[q]), -kk));
+          a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(YR(q)) + fabs(YI(q)), -kk));
         }
       }
       a = block_max(a, red);
@@ -318,8 +332,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
       if (e2 != 0) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-          yr[q] = ldexp(yr[q], e2);
-          yi[q] = ldexp(yi[q], e2);
+          YR(q) = ldexp(YR(q), e2);
+          YI(q) = ldexp(YI(q), e2);
         }
         qr = ldexp(qr, e2);
         qi = ldexp(qi, e2);
@@ -337,13 +351,13 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
         const double ri = fma(c, vi[q], fma(si, hr, -(sr * hi)));
         const double nvr = fma(c, hr, fma(sr, vr[q], -(si * vi[q])));
         const double nvi = fma(c, hi, fma(sr, vi[q], si * vr[q]));
-        yr[q] = fma(-qr, rr, fma(qi, ri, yr[q]));
-        yi[q] = fma(-qr, ri, fma(-qi, rr, yi[q]));
+        YR(q) = fma(-qr, rr, fma(qi, ri, YR(q)));
+        YI(q) = fma(-qr, ri, fma(-qi, rr, YI(q)));
         vr[q] = nvr;
         vi[q] = nvi;
       } else if (row == st) {
-        yr[q] = qr;
-        yi[q] = qi;
+        YR(q) = qr;
+        YI(q) = qi;
       }
     }
   };
@@ -356,7 +370,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
         load_col(st - D, ring[u], rsub[u]);   // the column of step st - D
         if (wave == ((st - 1) % kCThreads) >> 6)
           chain(st - 1, owner_value(vr, st - 1), owner_value(vi, st - 1),
-                owner_value(yr, st - 1), owner_value(yi, st - 1), rsub[(u + 1) % D], by);
+                owner_y(yrr, ylr, st - 1), owner_y(yir, yli, st - 1), rsub[(u + 1) % D], by);
         __syncthreads();
       }
     }
@@ -369,11 +383,11 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     for (int q = 0; q < RPT; ++q) {
       const int row = tid + kCThreads * q;
       if (row == 0) {
-        yr[q] = R.qr;
-        yi[q] = R.qi;
+        YR(q) = R.qr;
+        YI(q) = R.qi;
       } else if (R.e1 != 0) {
-        yr[q] = ldexp(yr[q], R.e1);
-        yi[q] = ldexp(yi[q], R.e1);
+        YR(q) = ldexp(YR(q), R.e1);
+        YI(q) = ldexp(YI(q), R.e1);
       }
     }
   }
@@ -390,8 +404,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     if (g0 < n) {
       const int row = g0 + tid;
       if (row < n) {
-        stg[0][tid] = yr[q];
-        stg[1][tid] = yi[q];
+        stg[0][tid] = YR(q);
+        stg[1][tid] = YI(q);
         stg[2][tid] = row >= 1 ? C[row] : 1.0;
         stg[3][tid] = row >= 1 ? SR[row] : 0.0;
         stg[4][tid] = row >= 1 ? SI[row] : 0.0;
@@ -417,12 +431,12 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
       }
       __syncthreads();
       if (tid + 1 < kCThreads && row + 1 < n) {   // produced at element tid + 1
-        yr[q] = xo[0][tid + 1];
-        yi[q] = xo[1][tid + 1];
+        YR(q) = xo[0][tid + 1];
+        YI(q) = xo[1][tid + 1];
       }
       if (q > 0 && tid == kCThreads - 1) {        // last row of the previous chunk
-        yr[q > 0 ? q - 1 : 0] = xo[0][0];
-        yi[q > 0 ? q - 1 : 0] = xo[1][0];
+        YR(q > 0 ? q - 1 : 0) = xo[0][0];
+        YI(q > 0 ? q - 1 : 0) = xo[1][0];
       }
       __syncthreads();
     }
@@ -435,15 +449,15 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
     if (tid + kCThreads * q == n - 1) {
-      yr[q] = xo[0][0];
-      yi[q] = xo[1][0];
+      YR(q) = xo[0][0];
+      YI(q) = xo[1][0];
     }
 
   // ||x||_2, the first entry of largest modulus, normalisation
   double mx = 0.0;
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
-    if (tid + kCThreads * q < n) mx = fmax(mx, fmax(fabs(yr[q]), fabs(yi[q])));
+    if (tid + kCThreads * q < n) mx = fmax(mx, fmax(fabs(YR(q)), fabs(YI(q))));
   mx = block_max(mx, red);
   double nrm = mx;
   int p = 0;
@@ -455,7 +469,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     for (int q = 0; q < RPT; ++q) {
       const int row = tid + kCThreads * q;
       if (row < n) {
-        const double a = ldexp(yr[q], -e), b = ldexp(yi[q], -e);
+        const double a = ldexp(YR(q), -e), b = ldexp(YI(q), -e);
         const double t = a * a + b * b;
         sum += t;
         if (t > best) {
@@ -474,8 +488,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
     if (tid + kCThreads * q == p) {
-      xo[0][0] = yr[q];
-      xo[1][0] = yi[q];
+      xo[0][0] = YR(q);
+      xo[1][0] = YI(q);
     }
   __syncthreads();
   double pr = 1.0, pi = 0.0;
@@ -491,17 +505,19 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   for (int q = 0; q < RPT; ++q) {
     const int row = tid + kCThreads * q;
     if (row < n) {
-      const double a = yr[q], b = yi[q];
+      const double a = YR(q), b = YI(q);
       outr[row] = scale ? (a * pr + b * pi) / nrm : a;
       outi[row] = scale ? (b * pr - a * pi) / nrm : b;
     }
   }
   if (tid == 0) A.status[k] = (conv ? HSRQ_CONVERGED : 0) | (info ? HSRQ_PERTURBED : 0);
+#undef YR
+#undef YI
 }
 
 template <int RPT>
 hipError_t launch_csolve(const CArgs& a, int m, hipStream_t st) {
-  hipLaunchKernelGGL(k_csolve<RPT>, dim3(m), dim3(kCThreads), 0, st, a);
+  hipLaunchKernelGGL((k_csolve<RPT, RPT == kCMaxRpt>), dim3(m), dim3(kCThreads), 0, st, a);
   return hipGetLastError();
 }
 

@@ -515,6 +515,299 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
 #undef YI
 }
 
+// Orders above 8192: the same sweep with the vector in global memory (per
+// shift 4 n doubles: Re v, Im v, Re y, Im y), rows strided over the threads
+// in runtime loops.  Every step reads and writes the active rows once (L2 /
+// HBM traffic instead of registers), so it is slower per row; results follow
+// the same equations and operation order as k_csolve.
+__global__ __launch_bounds__(kCThreads) void k_csolve_big(CArgs A, double* __restrict__ state) {
+  struct Rec {
+    double c, sr, si, qr, qi, by;
+    int e1, slow, info;
+  };
+  __shared__ Rec rec[2];
+  __shared__ double red[kCWaves];
+  __shared__ int redi[kCWaves];
+  __shared__ double stg[5][kCThreads];
+
+  const int k = blockIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int n = A.n;
+  const double* __restrict__ H = A.H;
+  const int64_t ldH = A.ldH;
+  const double sa = A.wr[k], sb = A.wi[k];
+  double* __restrict__ C = A.work + (int64_t)k * 3 * n;
+  double* __restrict__ SR = C + n;
+  double* __restrict__ SI = C + 2 * n;
+  double* __restrict__ VR = state + (int64_t)k * 4 * n;
+  double* __restrict__ VI = VR + n;
+  double* __restrict__ YR = VI + n;
+  double* __restrict__ YI = YR + n;
+
+  double hinf = 0.0;
+  for (int i = tid; i < n; i += kCThreads) hinf = fmax(hinf, A.rs[i]);
+  hinf = block_max(hinf, red);
+  const double eps = 0x1p-53;
+  const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));
+  const double asig = norm3(sa, sb, 0.0);
+  const double smin = fmax(eps * asig, smlnum);
+  const double eps3 = hinf * 0x1p-53;
+  const double slack = 1.0 + 0x1p-40;
+  const double hm = hinf + asig;
+  const double br = M_SQRT2 * hm * sqrt((double)n + 1.0) * slack;
+
+  for (int row = tid; row < n; row += kCThreads) {
+    double v = H[row + (int64_t)(n - 1) * ldH];
+    if (row == n - 1) v -= sa;
+    VR[row] = v;
+    VI[row] = (row == n - 1) ? -sb : 0.0;
+    YR[row] = eps3;
+    YI[row] = 0.0;
+  }
+
+  auto chain = [&](int st, double psr, double psi, double ysr, double ysi, double hsub,
+                   double by) {
+    double c = 1.0, sr = 0.0, si = 0.0, pr = 1.0, pi = 0.0, av, d;
+    {
+      const double h = st >= 1 ? hsub : 0.0;
+      const double mm = fmax(fabs(psr), fmax(fabs(psi), fabs(h)));
+      if (mm == 0.0 || !isfinite(mm)) {
+        av = fmax(fabs(psr), fabs(psi));
+        d = mm;
+      } else {
+        const int e = ilogb(mm);
+        const double a = ldexp(psr, -e), b = ldexp(psi, -e), g = ldexp(h, -e);
+        const double s2v = a * a + b * b;
+        av = ldexp(sqrt(s2v), e);
+        d = ldexp(sqrt(s2v + g * g), e);
+      }
+    }
+    const bool tiny = d < smin;
+    const double deff = tiny ? smin : d;
+    const double id = 1.0 / deff;
+    if (av > 0.0) {
+      const double ia = 1.0 / av;
+      pr = psr * ia;
+      pi = psi * ia;
+    }
+    if (st >= 1 && d > 0.0) {
+      const double idr = tiny ? 1.0 / d : id;
+      c = av * idr;
+      const double t = hsub * idr;
+      sr = -(t * pr);
+      si = t * pi;
+    }
+    const double qpr = tiny ? 1.0 : pr, qpi = tiny ? 0.0 : pi;
+    double wr = ysr * qpr + ysi * qpi;
+    double wi = ysi * qpr - ysr * qpi;
+    const int e1 = pow2_exp(protect_real_division(fabs(wr) + fabs(wi), deff));
+    if (e1 != 0) {
+      wr = ldexp(wr, e1);
+      wi = ldexp(wi, e1);
+      by = ldexp(by, e1);
+    }
+    const double qr = wr * id, qi = wi * id;
+    const double byn = (by + (fabs(qr) + fabs(qi)) * br) * slack;
+    if (lane == 0) {
+      Rec& o = rec[st & 1];
+      o.c = c;
+      o.sr = sr;
+      o.si = si;
+      o.qr = qr;
+      o.qi = qi;
+      o.by = byn;
+      o.e1 = e1;
+      o.slow = !(byn <= kOmega);
+      o.info = tiny ? 1 : 0;
+      if (st >= 1) {
+        C[st] = c;
+        SR[st] = sr;
+        SI[st] = si;
+      }
+    }
+  };
+  auto scale_y = [&](int e) {
+    for (int row = tid; row < n; row += kCThreads) {
+      YR[row] = ldexp(YR[row], e);
+      YI[row] = ldexp(YI[row], e);
+    }
+  };
+
+  int E = 0, info = 0;
+  double by = eps3;
+  if (wave == 0) {
+    // row n-1 holds its initial values, known to every thread
+    const double hsub = n >= 2 ? H[(n - 1) + (int64_t)(n - 2) * ldH] : 0.0;
+    chain(n - 1, H[(n - 1) + (int64_t)(n - 1) * ldH] - sa, -sb, eps3, 0.0, hsub, by);
+  }
+  __syncthreads();
+
+  for (int st = n - 1; st >= 1; --st) {
+    const Rec R = rec[st & 1];
+    const double c = R.c, sr = R.sr, si = R.si;
+    double qr = R.qr, qi = R.qi;
+    E += R.e1;
+    info |= R.info;
+    if (R.e1 != 0) scale_y(R.e1);   // each thread its own rows: no barrier needed
+    by = R.by;
+    const double* __restrict__ hc = H + (int64_t)(st - 1) * ldH;
+    if (R.slow) {
+      const double nq = fabs(qr) + fabs(qi);
+      const int kk = nq > 0.0 ? ilogb(nq) : 0;
+      const double nqs = ldexp(nq, -kk);
+      double a = 0.0;
+      for (int row = tid; row <= st - 1; row += kCThreads) {
+        const double hr = (row == st - 1) ? hc[row] - sa : hc[row];
+        const double hi = (row == st - 1) ? -sb : 0.0;
+        const double rr = c * VR[row] - (sr * hr + si * hi);
+        const double ri = c * VI[row] - (sr * hi - si * hr);
+        a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(YR[row]) + fabs(YI[row]), -kk));
+      }
+      a = block_max(a, red);
+      int e2 = 0;
+      if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
+      if (e2 != 0) {
+        scale_y(e2);
+        qr = ldexp(qr, e2);
+        qi = ldexp(qi, e2);
+        E += e2;
+      }
+      by = ldexp(a, kk + e2) * slack;
+    }
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;   // row st-1 after the update
+    for (int row = tid; row <= st - 1; row += kCThreads) {
+      const double hr = (row == st - 1) ? hc[row] - sa : hc[row];
+      const double hi = (row == st - 1) ? -sb : 0.0;
+      const double vr = VR[row], vi = VI[row];
+      const double rr = fma(c, vr, -fma(sr, hr, si * hi));
+      const double ri = fma(c, vi, fma(si, hr, -(sr * hi)));
+      const double nvr = fma(c, hr, fma(sr, vr, -(si * vi)));
+      const double nvi = fma(c, hi, fma(sr, vi, si * vr));
+      const double nyr = fma(-qr, rr, fma(qi, ri, YR[row]));
+      const double nyi = fma(-qr, ri, fma(-qi, rr, YI[row]));
+      VR[row] = nvr;
+      VI[row] = nvi;
+      YR[row] = nyr;
+      YI[row] = nyi;
+      if (row == st - 1) {
+        o0 = nvr;
+        o1 = nvi;
+        o2 = nyr;
+        o3 = nyi;
+      }
+    }
+    if (tid == st % kCThreads) {
+      YR[st] = qr;
+      YI[st] = qi;
+    }
+    if (wave == ((st - 1) % kCThreads) >> 6) {
+      const int ol = (st - 1) & 63;
+      const double hsub = st >= 2 ? H[(st - 1) + (int64_t)(st - 2) * ldH] : 0.0;
+      chain(st - 1, readlane(o0, ol), readlane(o1, ol), readlane(o2, ol), readlane(o3, ol), hsub,
+            by);
+    }
+    __syncthreads();
+  }
+  {
+    const Rec R = rec[0];
+    E += R.e1;
+    info |= R.info;
+    for (int row = tid; row < n; row += kCThreads) {
+      if (row == 0) {
+        YR[0] = R.qr;
+        YI[0] = R.qi;
+      } else if (R.e1 != 0) {
+        YR[row] = ldexp(YR[row], R.e1);
+        YI[row] = ldexp(YI[row], R.e1);
+      }
+    }
+  }
+  // every thread reads rows and rotations written by others below
+  __threadfence_block();
+  __syncthreads();
+
+  // x = U_{n-1} ... U_1 z in place of z: chunks staged in LDS, the recurrence
+  // by thread 0 (x_{i-1} overwrites z_{i-1}, already staged)
+  double tr = 0.0, ti = 0.0;
+  for (int g0 = 0; g0 < n; g0 += kCThreads) {
+    const int row = g0 + tid;
+    if (row < n) {
+      stg[0][tid] = YR[row];
+      stg[1][tid] = YI[row];
+      stg[2][tid] = row >= 1 ? C[row] : 1.0;
+      stg[3][tid] = row >= 1 ? SR[row] : 0.0;
+      stg[4][tid] = row >= 1 ? SI[row] : 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int cnt = min(kCThreads, n - g0);
+      for (int j = 0; j < cnt; ++j) {
+        const double zr = stg[0][j], zi = stg[1][j];
+        if (g0 + j == 0) {
+          tr = zr;
+          ti = zi;
+        } else {
+          const double cc = stg[2][j], s_r = stg[3][j], s_i = stg[4][j];
+          YR[g0 + j - 1] = cc * tr - (s_r * zr + s_i * zi);
+          YI[g0 + j - 1] = cc * ti - (s_r * zi - s_i * zr);
+          const double ntr = (s_r * tr - s_i * ti) + cc * zr;
+          const double nti = (s_r * ti + s_i * tr) + cc * zi;
+          tr = ntr;
+          ti = nti;
+        }
+      }
+      if (g0 + cnt == n) {
+        YR[n - 1] = tr;
+        YI[n - 1] = ti;
+      }
+    }
+    __syncthreads();
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  double mx = 0.0;
+  for (int row = tid; row < n; row += kCThreads) mx = fmax(mx, fmax(fabs(YR[row]), fabs(YI[row])));
+  mx = block_max(mx, red);
+  double nrm = mx;
+  int p = 0;
+  if (mx > 0.0 && isfinite(mx)) {
+    const int e = ilogb(mx);
+    double sum = 0.0, best = -1.0;
+    int bi = INT_MAX;
+    for (int row = tid; row < n; row += kCThreads) {
+      const double a = ldexp(YR[row], -e), b = ldexp(YI[row], -e);
+      const double t = a * a + b * b;
+      sum += t;
+      if (t > best) {
+        best = t;
+        bi = row;
+      }
+    }
+    sum = block_sum(sum, red);
+    const double bmax = block_max(best, red);
+    p = block_min_int(best == bmax ? bi : INT_MAX, redi);
+    nrm = ldexp(sqrt(sum), e);
+  }
+  const bool conv = isfinite(nrm) && nrm > ldexp(0.1 / sqrt((double)n), E);
+  double pr = 1.0, pi = 0.0;
+  const bool scale = nrm > 0.0 && isfinite(nrm);
+  if (scale) {
+    const double xr = YR[p], xi = YI[p];
+    const double ap = norm3(xr, xi, 0.0);
+    pr = xr / ap;
+    pi = xi / ap;
+  }
+  double* outr = A.vr + (int64_t)(2 * k) * A.ldvr;
+  double* outi = A.vr + (int64_t)(2 * k + 1) * A.ldvr;
+  for (int row = tid; row < n; row += kCThreads) {
+    const double a = YR[row], b = YI[row];
+    outr[row] = scale ? (a * pr + b * pi) / nrm : a;
+    outi[row] = scale ? (b * pr - a * pi) / nrm : b;
+  }
+  if (tid == 0) A.status[k] = (conv ? HSRQ_CONVERGED : 0) | (info ? HSRQ_PERTURBED : 0);
+}
+
 template <int RPT>
 hipError_t launch_csolve(const CArgs& a, int m, hipStream_t st) {
   hipLaunchKernelGGL((k_csolve<RPT, RPT == kCMaxRpt>), dim3(m), dim3(kCThreads), 0, st, a);
@@ -532,21 +825,26 @@ extern "C" int hsrq_solve_complex_device(hsrq_context* ctx, int n, const double*
   if (!ctx || n <= 0 || m < 0 || !dH || ldH < n || ldvr < n ||
       (m > 0 && (!dwr || !dwi || !dvr)))
     return HSRQ_EINVAL;
-  if (n > kCMaxN) return HSRQ_EUNSUPPORTED;
   if (m == 0) return HSRQ_OK;
   HIP_TRY(hipSetDevice(ctx->device));
-  // rotations (3 n per shift), row sums (n), their partial sums (64 n), status (m)
-  const size_t nd = 3 * (size_t)n * m + 65 * (size_t)n;
+  const bool big = n > kCMaxN;
+  // rotations (3 n per shift), row sums (n), their partial sums (64 n), the
+  // vectors of the large-order kernel (4 n per shift), status (m)
+  const size_t nd = 3 * (size_t)n * m + 65 * (size_t)n + (big ? 4 * (size_t)n * m : 0);
   char* buf = nullptr;
   HIP_TRY(hipMalloc(&buf, nd * sizeof(double) + (size_t)m * sizeof(int)));
   double* work = reinterpret_cast<double*>(buf);
   double* rs = work + 3 * (size_t)n * m;
   double* part = rs + n;
+  double* state = part + 64 * (size_t)n;
   int* dst = reinterpret_cast<int*>(buf + nd * sizeof(double));
   CArgs a{n, dH, ldH, dwr, dwi, dvr, ldvr, dst, work, rs};
   hipError_t e = launch_rowsums(n, dH, ldH, part, rs, ctx->stream);
   const int rpt = (n + kCThreads - 1) / kCThreads;
-  if (e == hipSuccess) {
+  if (e == hipSuccess && big) {
+    hipLaunchKernelGGL(k_csolve_big, dim3(m), dim3(kCThreads), 0, ctx->stream, a, state);
+    e = hipGetLastError();
+  } else if (e == hipSuccess) {
     if (rpt <= 1)
       e = launch_csolve<1>(a, m, ctx->stream);
     else if (rpt <= 2)

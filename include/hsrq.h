@@ -215,8 +215,9 @@ int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, i
  * column-major, column 2k the real and 2k+1 the imaginary part of x_k
  * (LAPACK's layout for a complex pair), ||x_k||_2 = 1, its first entry of
  * largest modulus real and positive.  status (host, m, may be NULL):
- * HSRQ_CONVERGED (growth test of dlaein), HSRQ_PERTURBED.  n <= 8192
- * (HSRQ_EUNSUPPORTED above).  The reference has the robust complex
+ * HSRQ_CONVERGED (growth test of dlaein), HSRQ_PERTURBED.  Up to n = 8192
+ * the vector stays on chip; above, it lives in HBM (slower per row).  Returns
+ * after the stream has drained.  The reference has the robust complex
  * primitives (robust.c:639-767) but no complex-shift solve; its driver
  * rejects complex eigenvalues (main.c:95-97). */
 int hsrq_solve_complex_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,

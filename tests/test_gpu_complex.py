@@ -66,10 +66,11 @@ def test_parity_complex_eigenvalues(ctx, n, seed):
         assert conv.all()
 
 
-@pytest.mark.parametrize("n", [1, 3000, 5000, 8192])
+@pytest.mark.parametrize("n", [1, 3000, 5000, 8192, 8193, 10000])
 def test_parity_far_shifts(ctx, n):
-    # sizes for every register layout (rows per thread 1 .. 16); shifts off
-    # the spectrum, so no eigenvalue computation is needed
+    # sizes for every register layout (rows per thread 1 .. 16) and the
+    # global-memory kernel above 8192; shifts off the spectrum, so no
+    # eigenvalue computation is needed
     h = w.gaussian_hessenberg(n, n)
     rng = np.random.default_rng(n)
     sig = rng.uniform(-3, 3, 6) * np.sqrt(n) + 1j * rng.uniform(0.1, 2.0, 6) * np.sqrt(n)
@@ -120,12 +121,46 @@ def test_graded(ctx):
 def test_rejects_and_empty(ctx):
     from hsrq3in_amd import lib
     L = lib()
-    n = 8193
     H = torch.zeros((1, 1), dtype=torch.float64, device="cuda")
-    assert L.hsrq_solve_complex_device(ctx.handle, n, H.data_ptr(), n, 1, H.data_ptr(),
-                                       H.data_ptr(), H.data_ptr(), n, None) == -4
     assert L.hsrq_solve_complex_device(ctx.handle, 5, H.data_ptr(), 5, 0, None, None, None, 5,
                                        None) == 0
     # leading dimension below n
     assert L.hsrq_solve_complex_device(ctx.handle, 5, H.data_ptr(), 4, 1, H.data_ptr(),
                                        H.data_ptr(), H.data_ptr(), 5, None) == -1
+
+
+def test_leading_dimensions(ctx):
+    # ldH > n and ldvr > n: the padding is neither read as data nor written
+    n, ld, ldv = 300, 307, 311
+    h, sig = eig_case(n, 4 + 300, 8)
+    x, st = gpu_complex(ctx, h, sig)
+    Ht = torch.full((n, ld), np.nan, dtype=torch.float64, device="cuda")
+    Ht[:, :n] = torch.from_numpy(np.ascontiguousarray(h.T)).cuda()
+    vrt = torch.full((2 * sig.size, ldv), 7.0, dtype=torch.float64, device="cuda")
+    wrt = torch.from_numpy(np.ascontiguousarray(sig.real)).cuda()
+    wit = torch.from_numpy(np.ascontiguousarray(sig.imag)).cuda()
+    vrt, st2 = ctx.solve_complex(Ht, wrt, wit, vrt)
+    v = vrt.cpu().numpy()
+    assert (v[:, n:] == 7.0).all()
+    assert np.array_equal((v[0::2, :n] + 1j * v[1::2, :n]).T, x)
+    assert np.array_equal(st, st2)
+
+
+@pytest.mark.parametrize("n", [8000, 9000])
+def test_known_spectrum_both_kernels(ctx, n):
+    # a prescribed real spectrum (no eigenvalue computation): real shifts as
+    # complex ones (wi = 0) on either side of the 8192 boundary equal the
+    # real path's vectors, and converge
+    from hsrq3in_amd import workloads as wl
+    lam = np.arange(1.0, n + 1.0)
+    h = wl.constructed_spectrum(lam, seed=n)
+    wr = lam[[0, 1, n // 2, n - 1]] * (1 + 1e-12)
+    x, st = gpu_complex(ctx, h, wr + 0j)
+    assert (st & HSRQ_CONVERGED).all() and np.all(x.imag == 0.0)
+    Ht = colmajor_to_device(h)
+    wrt = torch.from_numpy(wr.copy()).cuda()
+    vrt = torch.zeros((wr.size, n), dtype=torch.float64, device="cuda")
+    ctx.solve(Ht, wrt, vrt, 128)
+    xr = vrt.cpu().numpy().T
+    c = np.abs(np.sum(x.real * xr, axis=0)) / np.linalg.norm(xr, axis=0)
+    assert (1.0 - c).max() <= TOL

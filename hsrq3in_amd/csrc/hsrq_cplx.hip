@@ -4,9 +4,10 @@
 // not produce.  The reference ships the robust complex primitives but no
 // caller (robust.c:639-671 robust_cmplx_div, :677-767 solve_1x1_cmplx_system).
 //
-// One workgroup of 512 threads per shift, the whole vector in registers:
-// thread t holds rows t + 512 q, q < RPT (n <= 512 RPT <= 8192).  The sweep
-// runs right to left over the columns of H - sigma I:
+// One workgroup of 512 threads per shift, the whole vector on chip: thread t
+// holds rows t + 512 q, q < RPT (n <= 512 RPT <= 8192), in registers (y in
+// LDS for RPT = 16); above n = 8192 k_csolve_big keeps the vector in global
+// memory.  The sweep runs right to left over the columns of H - sigma I:
 //   * the running column v (column st after the rotations so far) and the
 //     next column h = H(:, st-1) (- sigma on the diagonal) are combined by the
 //     unitary M = [[c, -conj(s)], [s, c]] that zeroes row st of the new column

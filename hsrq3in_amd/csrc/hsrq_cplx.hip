@@ -23,8 +23,9 @@
 //     bound is reduced over the workgroup);
 //   * x = U_{n-1} ... U_1 z from the stored (c, s): a sequential complex
 //     recurrence over LDS-staged chunks.
-// Every step is one barrier: the thread owning row st-1 publishes its new
-// (v, y) in LDS for the next rotation, which every thread computes alike.
+// Every step is one barrier: after its row updates, the wavefront owning row
+// st-1 computes the next step's scalar chain (rotation, quotient, bounds)
+// from that row and publishes it in LDS; every thread applies it next step.
 // Output: column 2k of vr the real part, 2k+1 the imaginary part of x_k,
 // ||x_k||_2 = 1 with its first entry of largest modulus real and positive;
 // status bit 1 converged (dlaein's growth test, as the real path), 2 a pivot

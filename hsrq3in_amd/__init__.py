@@ -124,7 +124,7 @@ def lib():
         L.hsrq_grade_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, _ip, ctypes.c_int,
                                         _ip, _ip]
         L.hsrq_solve_complex_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
-                                                _vp, _vp, _vp, ctypes.c_int, _ip]
+                                                _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _ip]
         L.hsrq_backtransform_device.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                                 _vp, ctypes.c_int, _vp, ctypes.c_int]
         _lib = L
@@ -305,10 +305,11 @@ class Context:
         return res[:m]
 
 
-    def solve_complex(self, Ht, wrt, wit, vrt=None):
+    def solve_complex(self, Ht, wrt, wit, vrt=None, max_restarts: int = 0):
         """Complex shifts wr + i wi (device tensors, m entries) on Ht (as in
         solve).  Returns (vrt, status): vrt (2m, n) with rows 2k / 2k+1 the real
-        / imaginary parts of x_k (hsrq_solve_complex_device)."""
+        / imaginary parts of x_k (hsrq_solve_complex_device); unconverged shifts
+        are retried with dlaein's restart vectors up to max_restarts times."""
         import torch
         n, m = Ht.shape[0], wrt.numel()
         if vrt is None:
@@ -317,7 +318,8 @@ class Context:
         torch.cuda.synchronize(Ht.device)
         _check(lib().hsrq_solve_complex_device(self._h, n, Ht.data_ptr(), Ht.shape[1], m,
                                                wrt.data_ptr(), wit.data_ptr(), vrt.data_ptr(),
-                                               vrt.shape[1], st.ctypes.data_as(_ip)),
+                                               vrt.shape[1], int(max_restarts),
+                                               st.ctypes.data_as(_ip)),
                "hsrq_solve_complex_device")
         return vrt, st[:m]
 

@@ -33,7 +33,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "hsrq_device.h"
 #include "hsrq_host.h"
@@ -57,7 +59,24 @@ struct CArgs {
   int* status;
   double* work;        // 3 n per shift: c, Re s, Im s of rotation st at index st
   const double* rs;    // row sums ||H(i,:)||_1
+  int its;             // start vector: -1 eps3 (1, ..., 1); >= 0 the its-th restart vector
+  const int* idx;      // shift of each workgroup (nullptr: workgroup k solves shift k)
 };
+
+// Entry `row` of the start vector (inverse_iteration.c:19-57, dlaein's choice):
+// eps3 everywhere, or for restart its: eps3 / (sqrt(n) + 1) with eps3 in row 0
+// and eps3 sqrt(n) subtracted in row n-1-its.
+__device__ __forceinline__ double start_y(int row, int n, int its, double eps3) {
+  if (its < 0) return eps3;
+  const double sq = sqrt((double)n);
+  double y = (row == 0) ? eps3 : eps3 / (sq + 1.0);
+  if (row == n - 1 - its) y -= eps3 * sq;
+  return y;
+}
+// A bound of |y_i| for that start vector.
+__device__ __forceinline__ double start_bound(int n, int its, double eps3) {
+  return its < 0 ? eps3 : eps3 * (1.0 + sqrt((double)n));
+}
 
 // sqrt(a^2 + b^2 + c^2) with power-of-two scaling.
 __device__ __forceinline__ double norm3(double a, double b, double c) {
@@ -143,7 +162,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   __shared__ double stg[5][kCThreads];   // back-transform chunk: Re z, Im z, c, Re s, Im s
   __shared__ double xo[2][kCThreads + 1];
 
-  const int k = blockIdx.x;
+  const int k = A.idx ? A.idx[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   const int n = A.n;
   const double* __restrict__ H = A.H;
@@ -195,7 +214,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     const bool in = row < n;
     vr[q] = in ? H[row + (int64_t)(n - 1) * ldH] : 0.0;
     vi[q] = 0.0;
-    YR(q) = in ? eps3 : 0.0;
+    YR(q) = in ? start_y(row, n, A.its, eps3) : 0.0;
     YI(q) = 0.0;
     if (row == n - 1) {
       vr[q] -= sa;
@@ -292,7 +311,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   };
 
   int E = 0, info = 0;
-  double by = eps3;
+  double by = start_bound(n, A.its, eps3);
   if (wave == ((n - 1) % kCThreads) >> 6)
     chain(n - 1, owner_value(vr, n - 1), owner_value(vi, n - 1), owner_y(yrr, ylr, n - 1),
           owner_y(yir, yli, n - 1), rsub[0], by);
@@ -532,7 +551,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve_big(CArgs A, double* __res
   __shared__ int redi[kCWaves];
   __shared__ double stg[5][kCThreads];
 
-  const int k = blockIdx.x;
+  const int k = A.idx ? A.idx[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int n = A.n;
   const double* __restrict__ H = A.H;
@@ -563,7 +582,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve_big(CArgs A, double* __res
     if (row == n - 1) v -= sa;
     VR[row] = v;
     VI[row] = (row == n - 1) ? -sb : 0.0;
-    YR[row] = eps3;
+    YR[row] = start_y(row, n, A.its, eps3);
     YI[row] = 0.0;
   }
 
@@ -636,11 +655,12 @@ __global__ __launch_bounds__(kCThreads) void k_csolve_big(CArgs A, double* __res
   };
 
   int E = 0, info = 0;
-  double by = eps3;
+  double by = start_bound(n, A.its, eps3);
   if (wave == 0) {
     // row n-1 holds its initial values, known to every thread
     const double hsub = n >= 2 ? H[(n - 1) + (int64_t)(n - 2) * ldH] : 0.0;
-    chain(n - 1, H[(n - 1) + (int64_t)(n - 1) * ldH] - sa, -sb, eps3, 0.0, hsub, by);
+    chain(n - 1, H[(n - 1) + (int64_t)(n - 1) * ldH] - sa, -sb, start_y(n - 1, n, A.its, eps3),
+          0.0, hsub, by);
   }
   __syncthreads();
 
@@ -823,46 +843,66 @@ using namespace hsrq;
 
 extern "C" int hsrq_solve_complex_device(hsrq_context* ctx, int n, const double* dH, int ldH,
                                          int m, const double* dwr, const double* dwi,
-                                         double* dvr, int ldvr, int* status) {
-  if (!ctx || n <= 0 || m < 0 || !dH || ldH < n || ldvr < n ||
+                                         double* dvr, int ldvr, int max_restarts, int* status) {
+  if (!ctx || n <= 0 || m < 0 || !dH || ldH < n || ldvr < n || max_restarts < 0 ||
       (m > 0 && (!dwr || !dwi || !dvr)))
     return HSRQ_EINVAL;
   if (m == 0) return HSRQ_OK;
   HIP_TRY(hipSetDevice(ctx->device));
   const bool big = n > kCMaxN;
   // rotations (3 n per shift), row sums (n), their partial sums (64 n), the
-  // vectors of the large-order kernel (4 n per shift), status (m)
+  // vectors of the large-order kernel (4 n per shift); status and the shift
+  // list of a restart pass (m each)
   const size_t nd = 3 * (size_t)n * m + 65 * (size_t)n + (big ? 4 * (size_t)n * m : 0);
   char* buf = nullptr;
-  HIP_TRY(hipMalloc(&buf, nd * sizeof(double) + (size_t)m * sizeof(int)));
+  HIP_TRY(hipMalloc(&buf, nd * sizeof(double) + 2 * (size_t)m * sizeof(int)));
   double* work = reinterpret_cast<double*>(buf);
   double* rs = work + 3 * (size_t)n * m;
   double* part = rs + n;
   double* state = part + 64 * (size_t)n;
   int* dst = reinterpret_cast<int*>(buf + nd * sizeof(double));
-  CArgs a{n, dH, ldH, dwr, dwi, dvr, ldvr, dst, work, rs};
-  hipError_t e = launch_rowsums(n, dH, ldH, part, rs, ctx->stream);
+  int* didx = dst + m;
+  CArgs a{n, dH, ldH, dwr, dwi, dvr, ldvr, dst, work, rs, -1, nullptr};
   const int rpt = (n + kCThreads - 1) / kCThreads;
-  if (e == hipSuccess && big) {
-    hipLaunchKernelGGL(k_csolve_big, dim3(m), dim3(kCThreads), 0, ctx->stream, a, state);
-    e = hipGetLastError();
-  } else if (e == hipSuccess) {
-    if (rpt <= 1)
-      e = launch_csolve<1>(a, m, ctx->stream);
-    else if (rpt <= 2)
-      e = launch_csolve<2>(a, m, ctx->stream);
-    else if (rpt <= 4)
-      e = launch_csolve<4>(a, m, ctx->stream);
-    else if (rpt <= 8)
-      e = launch_csolve<8>(a, m, ctx->stream);
-    else
-      e = launch_csolve<16>(a, m, ctx->stream);
+  auto pass = [&](int count) -> hipError_t {
+    if (big) {
+      hipLaunchKernelGGL(k_csolve_big, dim3(count), dim3(kCThreads), 0, ctx->stream, a, state);
+      return hipGetLastError();
+    }
+    if (rpt <= 1) return launch_csolve<1>(a, count, ctx->stream);
+    if (rpt <= 2) return launch_csolve<2>(a, count, ctx->stream);
+    if (rpt <= 4) return launch_csolve<4>(a, count, ctx->stream);
+    if (rpt <= 8) return launch_csolve<8>(a, count, ctx->stream);
+    return launch_csolve<16>(a, count, ctx->stream);
+  };
+  std::vector<int> st(m, 0), todo;
+  hipError_t e = launch_rowsums(n, dH, ldH, part, rs, ctx->stream);
+  if (e == hipSuccess) e = pass(m);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(st.data(), dst, (size_t)m * sizeof(int), hipMemcpyDeviceToHost,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  // restarts (dlaein): the unconverged shifts again, with the its-th start
+  // vector, until they converge or max_restarts passes (at most n) are done
+  for (int its = 0; e == hipSuccess && its < std::min(max_restarts, n); ++its) {
+    todo.clear();
+    for (int k = 0; k < m; ++k)
+      if (!(st[k] & HSRQ_CONVERGED)) todo.push_back(k);
+    if (todo.empty()) break;
+    e = hipMemcpyAsync(didx, todo.data(), todo.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream);
+    a.its = its;
+    a.idx = didx;
+    if (e == hipSuccess) e = pass((int)todo.size());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(st.data(), dst, (size_t)m * sizeof(int), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   }
-  if (e == hipSuccess && status)
-    e = hipMemcpyAsync(status, dst, (size_t)m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
   const hipError_t es = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess) e = es;
   (void)hipFree(buf);
   HIP_TRY(e);
+  if (status) std::copy(st.begin(), st.end(), status);
   return HSRQ_OK;
 }

@@ -211,7 +211,9 @@ int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, i
 /* ---- complex shifts: eigenvectors of complex-conjugate eigenpairs ----
  * sigma_k = wr[k] + i wi[k] (device arrays, m entries).  One inverse-iteration
  * solve (H - sigma_k I) x = eps3 (1, ..., 1) per shift with complex Givens
- * rotations and power-of-two protections; no restarts.  dvr: n x 2m device
+ * rotations and power-of-two protections.  Shifts that fail the growth test
+ * are solved again with dlaein's restart vectors, at most max_restarts times
+ * (0: none; n: dlaein's limit); shifts keep their positions.  dvr: n x 2m device
  * column-major, column 2k the real and 2k+1 the imaginary part of x_k
  * (LAPACK's layout for a complex pair), ||x_k||_2 = 1, its first entry of
  * largest modulus real and positive.  status (host, m, may be NULL):
@@ -222,7 +224,7 @@ int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, i
  * rejects complex eigenvalues (main.c:95-97). */
 int hsrq_solve_complex_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
                               const double *dwr, const double *dwi, double *dvr, int ldvr,
-                              int *status);
+                              int max_restarts, int *status);
 
 /* ---- test-problem generation on the device (the step before the solve) ----
  * The reference builds its problem on the host (problemgenerator.c:305-356:

@@ -61,7 +61,7 @@ def lib():
         L.oracle_complex_inverse_iteration.restype = ctypes.c_int
         L.oracle_complex_inverse_iteration.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int,
-            ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+            ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -137,9 +137,10 @@ def residuals(H: np.ndarray, wr: np.ndarray, X: np.ndarray, nthreads: int = 0):
     return ns, ref
 
 
-def complex_inverse_iteration(H: np.ndarray, wr, wi, nthreads: int = 0):
+def complex_inverse_iteration(H: np.ndarray, wr, wi, nthreads: int = 0, max_restarts: int = 0):
     """Complex shifts wr + i wi: returns (X complex n x m, status int array);
-    X[:, k] unit 2-norm with its first entry of largest modulus real positive."""
+    X[:, k] unit 2-norm with its first entry of largest modulus real positive;
+    unconverged shifts retried with dlaein's restart vectors up to max_restarts times."""
     H = np.asfortranarray(H, dtype=np.float64)
     n = H.shape[0]
     wr = np.ascontiguousarray(wr, dtype=np.float64)
@@ -148,7 +149,8 @@ def complex_inverse_iteration(H: np.ndarray, wr, wi, nthreads: int = 0):
     vr = np.zeros((n, 2 * max(m, 1)), dtype=np.float64, order="F")
     st = np.zeros(max(m, 1), dtype=np.int32)
     rc = lib().oracle_complex_inverse_iteration(n, _ptr(H), n, m, _ptr(wr), _ptr(wi), _ptr(vr),
-                                                n, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                n, max_restarts,
+                                                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                 nthreads)
     if rc != 0:
         raise ValueError("oracle_complex_inverse_iteration: invalid arguments")

@@ -956,7 +956,7 @@ static cpivot_t cpivot(int st, double vr, double vi, double hsub, double yr, dou
 /* One complex shift; x (2n: re, im) unit 2-norm with its largest entry real
  * positive.  Returns status bits (1 converged, 2 perturbed). */
 static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, double si0,
-                      double *xr, double *xi, double *C, double *SR, double *SI)
+                      int its, double *xr, double *xi, double *C, double *SR, double *SI)
 {
     const double eps = 0x1p-53;
     const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));
@@ -971,16 +971,24 @@ static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, 
     double *vr = (double *)malloc(sizeof(double) * (size_t)n);
     double *vi = (double *)malloc(sizeof(double) * (size_t)n);
     double *yr = xr, *yi = xi;   /* y lives in x until the back-transform */
+    /* start vector (inverse_iteration.c:19-57): eps3 (1, ..., 1), or dlaein's
+     * its-th restart vector */
+    const double sq = sqrt((double)n);
     for (int i = 0; i < n; i++) {
         vr[i] = H[i + (size_t)(n - 1) * ldH];
         vi[i] = 0.0;
-        yr[i] = eps3;
+        if (its < 0) {
+            yr[i] = eps3;
+        } else {
+            yr[i] = (i == 0) ? eps3 : eps3 / (sq + 1.0);
+            if (i == n - 1 - its) yr[i] -= eps3 * sq;
+        }
         yi[i] = 0.0;
     }
     vr[n - 1] -= sr0;
     vi[n - 1] = -si0;
     int E = 0, info = 0;
-    double by = eps3;
+    double by = its < 0 ? eps3 : eps3 * (1.0 + sq);
     for (int st = n - 1; st >= 1; st--) {
         const cpivot_t P = cpivot(st, vr[st], vi[st], H[st + (size_t)(st - 1) * ldH], yr[st],
                                   yi[st], smin);
@@ -1094,10 +1102,11 @@ static int csolve_one(int n, const double *H, int ldH, double hinf, double sr0, 
 }
 
 int oracle_complex_inverse_iteration(int n, const double *H, int ldH, int m, const double *wr,
-                                     const double *wi, double *vr, int ldvr, int *status,
-                                     int nthreads)
+                                     const double *wi, double *vr, int ldvr, int max_restarts,
+                                     int *status, int nthreads)
 {
-    if (n <= 0 || ldH < n || m < 0 || ldvr < n || !H || (m > 0 && (!wr || !wi || !vr)))
+    if (n <= 0 || ldH < n || m < 0 || ldvr < n || !H || max_restarts < 0 ||
+        (m > 0 && (!wr || !wi || !vr)))
         return -1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1110,8 +1119,15 @@ int oracle_complex_inverse_iteration(int n, const double *H, int ldH, int m, con
         double *w = (double *)malloc(sizeof(double) * 3 * (size_t)n);
 #pragma omp for schedule(dynamic, 1)
         for (int k = 0; k < m; k++) {
-            const int st = csolve_one(n, H, ldH, hinf, wr[k], wi[k], vr + (size_t)2 * k * ldvr,
-                                      vr + (size_t)(2 * k + 1) * ldvr, w, w + n, w + 2 * n);
+            /* dlaein: restart vectors until the growth test passes, at most
+             * max_restarts times (and n) */
+            const int lim = max_restarts < n ? max_restarts : n;
+            int st = 0;
+            for (int its = -1; its < lim; its++) {
+                st = csolve_one(n, H, ldH, hinf, wr[k], wi[k], its, vr + (size_t)2 * k * ldvr,
+                                vr + (size_t)(2 * k + 1) * ldvr, w, w + n, w + 2 * n);
+                if (st & 1) break;
+            }
             if (status) status[k] = st;
         }
         free(w);

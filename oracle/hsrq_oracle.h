@@ -53,14 +53,16 @@ void oracle_residuals(int n, const double *H, int ldH, int m, const double *wr,
                       const double *X, int ldX, double *ns, double *ref,
                       int nthreads);
 
-/* Complex shifts sigma_k = wr[k] + i wi[k]: one inverse-iteration solve
- * (H - sigma_k I) x = eps3 (1, ..., 1) per shift, no restarts.  vr: n x 2m,
- * column 2k the real and 2k+1 the imaginary part of x_k, ||x_k||_2 = 1 and
- * its first entry of largest modulus real positive.  status (may be NULL):
- * 1 converged (growth test of dlaein), 2 a pivot raised to smin. */
+/* Complex shifts sigma_k = wr[k] + i wi[k]: inverse iteration
+ * (H - sigma_k I) x = eps3 (1, ..., 1) per shift, then dlaein's restart
+ * vectors while the growth test fails, at most max_restarts times.  vr:
+ * n x 2m, column 2k the real and 2k+1 the imaginary part of x_k,
+ * ||x_k||_2 = 1 and its first entry of largest modulus real positive.  status
+ * (may be NULL): 1 converged (growth test of dlaein), 2 a pivot raised to
+ * smin. */
 int oracle_complex_inverse_iteration(int n, const double *H, int ldH, int m,
                                      const double *wr, const double *wi, double *vr,
-                                     int ldvr, int *status, int nthreads);
+                                     int ldvr, int max_restarts, int *status, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -30,11 +30,11 @@ def ctx():
     c.close()
 
 
-def gpu_complex(ctx, h, sig):
+def gpu_complex(ctx, h, sig, max_restarts=0):
     Ht = colmajor_to_device(h)
     wrt = torch.from_numpy(np.ascontiguousarray(sig.real)).cuda()
     wit = torch.from_numpy(np.ascontiguousarray(sig.imag)).cuda()
-    vrt, st = ctx.solve_complex(Ht, wrt, wit)
+    vrt, st = ctx.solve_complex(Ht, wrt, wit, max_restarts=max_restarts)
     v = vrt.cpu().numpy()
     return (v[0::2] + 1j * v[1::2]).T, st
 
@@ -123,10 +123,12 @@ def test_rejects_and_empty(ctx):
     L = lib()
     H = torch.zeros((1, 1), dtype=torch.float64, device="cuda")
     assert L.hsrq_solve_complex_device(ctx.handle, 5, H.data_ptr(), 5, 0, None, None, None, 5,
-                                       None) == 0
-    # leading dimension below n
+                                       0, None) == 0
+    # leading dimension below n; negative restart count
     assert L.hsrq_solve_complex_device(ctx.handle, 5, H.data_ptr(), 4, 1, H.data_ptr(),
-                                       H.data_ptr(), H.data_ptr(), 5, None) == -1
+                                       H.data_ptr(), H.data_ptr(), 5, 0, None) == -1
+    assert L.hsrq_solve_complex_device(ctx.handle, 1, H.data_ptr(), 1, 1, H.data_ptr(),
+                                       H.data_ptr(), H.data_ptr(), 1, -1, None) == -1
 
 
 def test_leading_dimensions(ctx):
@@ -164,3 +166,27 @@ def test_known_spectrum_both_kernels(ctx, n):
     xr = vrt.cpu().numpy().T
     c = np.abs(np.sum(x.real * xr, axis=0)) / np.linalg.norm(xr, axis=0)
     assert (1.0 - c).max() <= TOL
+
+
+@pytest.mark.parametrize("n", [40, 700, 9000])
+def test_restart_vectors(ctx, n):
+    # shifts off the spectrum never pass the growth test: after r restarts the
+    # vector is the solve from the (r-1)-th restart vector, in both codes;
+    # converged shifts are not solved again
+    h = w.gaussian_hessenberg(n, 77)
+    rng = np.random.default_rng(3)
+    far = rng.uniform(1, 2, 3) * np.sqrt(n) * 3 + 1j * rng.uniform(0.1, 1, 3) * np.sqrt(n)
+    if n <= 700:
+        ev = np.linalg.eigvals(h)
+        near = ev[ev.imag > 0][:3] * (1 + 1e-13)
+        sig = np.concatenate([far, near])
+    else:
+        sig = far
+    x, st = gpu_complex(ctx, h, sig, max_restarts=3)
+    xo, sto = oracle.complex_inverse_iteration(h, sig.real, sig.imag, max_restarts=3)
+    assert np.array_equal(st, sto)
+    assert np.abs(x - xo).max() <= TOL
+    x0, _ = gpu_complex(ctx, h, sig)
+    conv = (st & HSRQ_CONVERGED) == HSRQ_CONVERGED
+    assert np.array_equal(x[:, conv], x0[:, conv])
+    assert not np.array_equal(x[:, ~conv], x0[:, ~conv])

@@ -136,3 +136,21 @@ def test_exact_eigenvalue_of_triangular_matrix():
     e0 = np.zeros(n)
     e0[0] = 1.0
     assert np.abs(x[:, 0] - e0).max() <= 1e-12
+
+
+def test_restart_vector_matches_dense_solve():
+    # a shift off the spectrum fails the growth test every time: with r
+    # restarts the result is the solve from dlaein's (r-1)-th restart vector
+    n = 90
+    h = w.gaussian_hessenberg(n, 4)
+    sig = np.array([40.0 + 7.0j])
+    eps3 = np.linalg.norm(h, np.inf) * 2.0 ** -53
+    sq = np.sqrt(n)
+    for r in (1, 2, 5):
+        x, st = oracle.complex_inverse_iteration(h, sig.real, sig.imag, max_restarts=r)
+        assert st[0] & 1 == 0
+        y0 = np.full(n, eps3 / (sq + 1.0))
+        y0[0] = eps3
+        y0[n - 1 - (r - 1)] -= eps3 * sq
+        xd = np.linalg.solve(h - sig[0] * np.eye(n), y0.astype(complex))
+        assert np.abs(phase_normalise(xd) - x[:, 0]).max() <= 1e-12

@@ -47,7 +47,10 @@ enum {
 
 typedef struct hsrq_context hsrq_context;
 
-/* Context = device + stream + workspace cache.  device < 0: current device. */
+/* Context = device + stream + workspace cache (and, made on first use, the
+ * rocBLAS handle of the back-transformation).  device < 0: current device.
+ * One host thread at a time per context; use one context per thread.  The
+ * request server (below) is the thread-safe entry for concurrent callers. */
 int hsrq_context_create(hsrq_context **ctx, int device);
 void hsrq_context_destroy(hsrq_context *ctx);
 /* Tile size used when a call passes tlsz <= 0 (autotuner default: 0). */

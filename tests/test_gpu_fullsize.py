@@ -1,0 +1,97 @@
+"""Parity at the benchmark's full size: n = 8000, m = 800 (the Gaussian
+Hessenberg of seed 2 with its stored shifts), its graded variant
+(k in [-400, 400]), and the same problem with complex shifts.
+
+The oracle solves all 800 shifts in about a second on the host, so the bar is
+the one of the small parity tests: 1 - |cos| <= 1e-10 against the oracle
+(1e-8 graded), the same shift order and status, residual <= 10 n, no
+Inf/NaN; plus the size-independent properties: unit norms, identical
+results from a plan replayed as a HIP graph and from the plain device call.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+import oracle
+from hsrq3in_amd import HSRQ_CONVERGED, colmajor_to_device, workloads
+from tests.helpers import one_minus_abs_cos
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+N, M = 8000, 800
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hsrq3in_amd import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+@functools.lru_cache(maxsize=None)
+def problem(graded: bool):
+    H = workloads.gaussian_hessenberg(N, 2)
+    if graded:
+        H = workloads.graded(H, 400, 2)
+    return H, workloads.load_shifts("shifts_n8000_s2.npz")
+
+
+def device_solve(ctx, H, wr):
+    Ht = colmajor_to_device(H)
+    wrt = torch.from_numpy(wr.copy()).cuda()
+    vrt = torch.zeros((M, N), dtype=torch.float64, device="cuda")
+    st = ctx.solve(Ht, wrt, vrt, 128)
+    res = ctx.residuals(Ht, wrt, vrt)
+    return Ht, vrt, wrt.cpu().numpy(), st, res
+
+
+@pytest.mark.parametrize("graded", [False, True], ids=["plain", "graded"])
+def test_fullsize_against_oracle(ctx, graded):
+    H, wr = problem(graded)
+    Ht, vrt, wo, st, res = device_solve(ctx, H, wr)
+    x = np.asfortranarray(vrt.cpu().numpy().T)
+    xo, woo, sto = oracle.tiled_inverse_iteration(H, wr, 128, 16)
+    assert np.isfinite(x).all()
+    assert np.array_equal(wo, woo)
+    assert (st & HSRQ_CONVERGED).all() and sto.nonconverged == 0
+    assert res.max() <= 10 * N
+    np.testing.assert_allclose(np.linalg.norm(x, axis=0), 1.0, rtol=1e-13)
+    d = one_minus_abs_cos(x, xo)
+    print(f"n=8000 m=800 {'graded' if graded else 'plain'}: max 1-|cos| {d.max():.2e}, "
+          f"max residual {res.max():.1f}")
+    assert d.max() <= (1e-8 if graded else 1e-10)
+
+
+def test_fullsize_graph_replay_equals_device_call(ctx):
+    H, wr = problem(False)
+    Ht, vrt, wo, st, _ = device_solve(ctx, H, wr)
+    wrt = torch.from_numpy(wr.copy()).cuda()
+    v2 = torch.zeros_like(vrt)
+    plan = ctx.plan([(Ht, wrt, v2)], 128)
+    for _ in range(2):
+        v2.zero_()
+        plan.execute(graph=True)
+        plan.sync()
+        assert torch.equal(v2, vrt)
+    plan.close()
+
+
+def test_fullsize_complex_shifts(ctx):
+    # shifts in the spectral disc (the Gaussian matrix's complex eigenvalues
+    # would need an O(n^3) host eigensolver): parity with the complex oracle
+    H, _ = problem(False)
+    rng = np.random.default_rng(8)
+    rad = np.sqrt(N) * np.sqrt(rng.uniform(0.0, 1.0, 24))
+    ang = rng.uniform(0.05, np.pi - 0.05, 24)
+    sig = rad * np.exp(1j * ang)
+    Ht = colmajor_to_device(H)
+    vrt, st = ctx.solve_complex(Ht, torch.from_numpy(sig.real.copy()).cuda(),
+                                torch.from_numpy(sig.imag.copy()).cuda())
+    v = vrt.cpu().numpy()
+    x = (v[0::2] + 1j * v[1::2]).T
+    xo, sto = oracle.complex_inverse_iteration(H, sig.real, sig.imag)
+    assert np.isfinite(x).all() and np.array_equal(st, sto)
+    assert np.abs(x - xo).max() <= 1e-10

@@ -122,31 +122,6 @@ __device__ __forceinline__ int block_min_int(int v, int* redi) {
   return r;
 }
 
-// y / (ph d) with the pivot raised to smin below it and the division
-// protected by 2^e1 (applied to the quotient here, to the other entries by
-// the caller).
-__device__ __forceinline__ int cdiv_pivot(double smin, double d, double pr, double pi,
-                                          double yr, double yi, double& qr, double& qi,
-                                          int& e1) {
-  int info = 0;
-  if (d < smin) {
-    d = smin;
-    pr = 1.0;
-    pi = 0.0;
-    info = 1;
-  }
-  double wr = yr * pr + yi * pi;
-  double wi = yi * pr - yr * pi;
-  e1 = pow2_exp(protect_real_division(fabs(wr) + fabs(wi), d));
-  if (e1 != 0) {
-    wr = ldexp(wr, e1);
-    wi = ldexp(wi, e1);
-  }
-  qr = wr / d;
-  qi = wi / d;
-  return info;
-}
-
 // YL: the right-hand side y lives in LDS instead of registers (the layout of
 // 16 rows per thread, whose registers would otherwise spill).
 template <int RPT, bool YL>

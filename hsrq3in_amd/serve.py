@@ -136,9 +136,10 @@ class Server:
 #
 # Request file  (*.hsrq, little-endian):
 #   8 bytes  magic "HSRQREQ1"
-#   4 x int32  n, m, flags (bit 0: complex shifts), reserved (0)
+#   4 x int32  n, m, flags (bit 0: complex shifts; bit 1: no shifts given,
+#              use the first m real eigenvalues of H), reserved (0)
 #   n*n float64  H, column-major (leading dimension n)
-#   m float64    wr
+#   m float64    wr              (absent with flag bit 1)
 #   m float64    wi              (only with the complex flag)
 # Response file (*.hsrs):
 #   8 bytes  magic "HSRQRES1"
@@ -155,15 +156,28 @@ class Server:
 REQ_MAGIC = b"HSRQREQ1"
 RES_MAGIC = b"HSRQRES1"
 FLAG_COMPLEX = 1
+FLAG_EIGENVALUE_SHIFTS = 2
 _HDR = np.dtype([("magic", "S8"), ("n", "<i4"), ("m", "<i4"), ("flags", "<i4"),
                  ("aux", "<i4")])
 
 
-def write_request(path: str, H: np.ndarray, wr, wi=None) -> None:
+def write_request(path: str, H: np.ndarray, wr, wi=None, m: int | None = None) -> None:
+    """wr (and wi): the shifts; or wr=None and m: the server takes the first m
+    real eigenvalues of H as shifts."""
     H = np.asfortranarray(H, dtype="<f8")
     n = H.shape[0]
     if H.shape != (n, n):
         raise ValueError("H must be square")
+    if wr is None:
+        if m is None or m < 0 or wi is not None:
+            raise ValueError("without shifts give m (and no wi)")
+        hdr = np.array([(REQ_MAGIC, n, m, FLAG_EIGENVALUE_SHIFTS, 0)], dtype=_HDR)
+        tmp = path + ".part"
+        with open(tmp, "wb") as f:
+            f.write(hdr.tobytes())
+            f.write(H.tobytes(order="F"))
+        os.replace(tmp, path)
+        return
     wr = np.ascontiguousarray(wr, dtype="<f8")
     flags = 0
     if wi is not None:
@@ -187,14 +201,24 @@ def _read_header(path: str, magic: bytes):
     if hdr.size != 1 or hdr["magic"][0] != magic:
         raise ValueError(f"{path}: not a {magic.decode()} file")
     n, m, flags, aux = (int(hdr[f][0]) for f in ("n", "m", "flags", "aux"))
-    if n <= 0 or m < 0 or flags & ~FLAG_COMPLEX:
+    if (n <= 0 or m < 0 or flags & ~(FLAG_COMPLEX | FLAG_EIGENVALUE_SHIFTS)
+            or flags == FLAG_COMPLEX | FLAG_EIGENVALUE_SHIFTS
+            or (magic == RES_MAGIC and flags & FLAG_EIGENVALUE_SHIFTS)):
         raise ValueError(f"{path}: invalid header (n={n}, m={m}, flags={flags})")
     return n, m, flags, aux
 
 
 def read_request(path: str):
-    """(H column-major, wr, wi or None); H is memory-mapped."""
+    """(H column-major, wr, wi or None); H is memory-mapped.  wr is None for a
+    request that asks for H's first m real eigenvalues as shifts (then the
+    fourth value is m)."""
     n, m, flags, _ = _read_header(path, REQ_MAGIC)
+    if flags & FLAG_EIGENVALUE_SHIFTS:
+        need = _HDR.itemsize + 8 * n * n
+        if os.path.getsize(path) != need:
+            raise ValueError(f"{path}: size {os.path.getsize(path)} != {need}")
+        H = np.memmap(path, dtype="<f8", mode="r", offset=_HDR.itemsize, shape=(n, n), order="F")
+        return H, None, None, m
     cplx = bool(flags & FLAG_COMPLEX)
     need = _HDR.itemsize + 8 * (n * n + m * (2 if cplx else 1))
     if os.path.getsize(path) != need:
@@ -204,7 +228,7 @@ def read_request(path: str):
     wr = np.fromfile(path, dtype="<f8", count=m, offset=off + 8 * n * n)
     wi = (np.fromfile(path, dtype="<f8", count=m, offset=off + 8 * (n * n + m))
           if cplx else None)
-    return H, wr, wi
+    return H, wr, wi, m
 
 
 def write_response(path: str, vr, wr, status, latency_ms: float, rc: int = 0,
@@ -246,12 +270,15 @@ def read_response(path: str):
 
 
 def serve_spool(ctx, in_dir: str, out_dir: str, hbm_bytes: int = 8 << 30,
-                max_active: int = 1024) -> dict:
+                max_active: int = 1024, shift_cache=None) -> dict:
     """Serves every *.hsrq request in in_dir that has no response yet: real
     shifts through the continuous-batching server (all admitted together),
-    complex shifts through hsrq_solve_complex_device.  Writes <name>.hsrs
-    into out_dir.  Returns {name: rc}."""
+    complex shifts through hsrq_solve_complex_device.  Requests without
+    shifts get H's first m real eigenvalues from ``shift_cache`` (a
+    workloads.ShiftCache; one in out_dir/.shifts by default).  Writes
+    <name>.hsrs into out_dir.  Returns {name: rc}."""
     import time
+    from .workloads import ShiftCache
     os.makedirs(out_dir, exist_ok=True)
     names = sorted(f[:-5] for f in os.listdir(in_dir) if f.endswith(".hsrq"))
     names = [b for b in names if not os.path.exists(os.path.join(out_dir, b + ".hsrs"))]
@@ -261,10 +288,21 @@ def serve_spool(ctx, in_dir: str, out_dir: str, hbm_bytes: int = 8 << 30,
         pending = []
         for b in names:
             try:
-                H, wr, wi = read_request(os.path.join(in_dir, b + ".hsrq"))
+                H, wr, wi, m = read_request(os.path.join(in_dir, b + ".hsrq"))
             except ValueError:
                 done[b] = -1
                 continue
+            if wr is None:
+                if shift_cache is None:
+                    shift_cache = ShiftCache(os.path.join(out_dir, ".shifts"))
+                try:
+                    wr = shift_cache.get(H, m)
+                except ValueError:   # fewer real eigenvalues than m
+                    n = H.shape[0]
+                    write_response(os.path.join(out_dir, b + ".hsrs"), np.zeros((n, m)),
+                                   np.zeros(m), np.zeros(m, np.int32), 0.0, -1)
+                    done[b] = -1
+                    continue
             if wi is not None:
                 t0 = time.perf_counter()
                 vr, wro, st, rc = _complex_request(ctx, H, wr, wi)

@@ -12,8 +12,9 @@ def test_request_round_trip(tmp_path):
     wr = np.linspace(-1, 1, 5)
     p = str(tmp_path / "a.hsrq")
     serve.write_request(p, H, wr)
-    H2, wr2, wi2 = serve.read_request(p)
+    H2, wr2, wi2, m2 = serve.read_request(p)
     assert np.array_equal(np.asarray(H2), H) and np.array_equal(wr2, wr) and wi2 is None
+    assert m2 == 5
     assert (tmp_path / "a.hsrq").stat().st_size == 24 + 8 * (37 * 37 + 5)
 
 
@@ -22,7 +23,7 @@ def test_complex_request_round_trip(tmp_path):
     wr, wi = np.array([0.5, -1.0]), np.array([2.0, 0.25])
     p = str(tmp_path / "c.hsrq")
     serve.write_request(p, H, wr, wi)
-    H2, wr2, wi2 = serve.read_request(p)
+    H2, wr2, wi2, _ = serve.read_request(p)
     assert np.array_equal(np.asarray(H2), H)
     assert np.array_equal(wr2, wr) and np.array_equal(wi2, wi)
 
@@ -62,3 +63,29 @@ def test_bad_files_rejected(tmp_path):
 def test_no_partial_files_left(tmp_path):
     serve.write_request(str(tmp_path / "a.hsrq"), np.eye(4), [1.0])
     assert sorted(f.name for f in tmp_path.iterdir()) == ["a.hsrq"]
+
+
+def test_request_without_shifts(tmp_path):
+    H = w.gaussian_hessenberg(30, 5)
+    p = str(tmp_path / "e.hsrq")
+    serve.write_request(p, H, None, m=4)
+    H2, wr2, wi2, m2 = serve.read_request(p)
+    assert np.array_equal(np.asarray(H2), H) and wr2 is None and wi2 is None and m2 == 4
+    assert (tmp_path / "e.hsrq").stat().st_size == 24 + 8 * 30 * 30
+    with pytest.raises(ValueError):
+        serve.write_request(p, H, None)                       # m missing
+
+
+def test_shift_cache(tmp_path):
+    H = w.gaussian_hessenberg(60, 6)
+    c = w.ShiftCache(str(tmp_path / "cache"))
+    a = c.get(H, 5)
+    assert np.array_equal(a, w.real_shifts(H, 5))
+    b = c.get(np.asfortranarray(H.copy()), 3)                # same matrix: a hit
+    assert np.array_equal(b, a[:3]) and (c.hits, c.misses) == (1, 1)
+    H2 = H.copy()
+    H2[0, 0] += 1e-9                                          # another matrix: a miss
+    c.get(H2, 1)
+    assert c.misses == 2 and len(list((tmp_path / "cache").iterdir())) == 2
+    with pytest.raises(ValueError):
+        c.get(H, 61)

@@ -532,7 +532,7 @@ def main():
         out["around_the_path"] = around_path(ctx)
     if not args.no_batch:
         ctx.set_packing(args.batch_packing)
-        out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 32)
+        out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 256)
         out["batched"]["packing"] = args.batch_packing
     if args.precision_study:
         out["precision"] = {"graded": precision_study(), "ungraded": precision_study(frac=0.0)}

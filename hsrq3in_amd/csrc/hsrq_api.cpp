@@ -986,7 +986,7 @@ int hsrq_autotune(hsrq_context* ctx, int count, const hsrq_problem* probs, int* 
   int bt = 0, bp = 0, bl = 0, rc = HSRQ_OK;
   static const int kTiles[] = {64, 128, 256};
   static const int kPacks[] = {1, 4};
-  static const int kLanes[] = {1, 2, 3};
+  static const int kLanes[] = {1, 2, 3, 4};
   for (int t : kTiles)
     for (int pk : kPacks)
       for (int ln : kLanes) {

@@ -500,8 +500,11 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     rare = rare | !(byn <= kOmega);
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
+      // r is 0 on the rows outside 0 .. st-1 (vstep), where the update
+      // leaves y as it is (yq is finite when this result is used)
       const int row = lane + 64 * q;
-      ny[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
+      const double u = fma(-yq, r[q], y[q]);
+      ny[q] = (row == st) ? yq : u;
     }
     return rare;
   };
@@ -935,11 +938,14 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     rare = rare | !(byn <= kOmega);
     return rare;
   };
+  // r is 0 on the rows outside 0 .. st-1 (vvec), where the update leaves y
+  // as it is (yq is finite on this path): one select per slot, for row st
   auto yvec = [&](int st, double yq, const double (&r)[RPL]) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int row = t + 16 * q;
-      y[q] = (row <= st - 1) ? fma(-yq, r[q], y[q]) : ((row == st) ? yq : y[q]);
+      const double u = fma(-yq, r[q], y[q]);
+      y[q] = (row == st) ? yq : u;
     }
   };
   auto ystep_slow = [&](int st, const VOut& o, const double (&r)[RPL]) __attribute__((always_inline)) {

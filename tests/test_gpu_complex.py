@@ -66,7 +66,7 @@ def test_parity_complex_eigenvalues(ctx, n, seed):
         assert conv.all()
 
 
-@pytest.mark.parametrize("n", [1, 3000, 5000, 8192, 8193, 10000])
+@pytest.mark.parametrize("n", [1, 1025, 2049, 3000, 4096, 4097, 5000, 8192, 8193, 10000])
 def test_parity_far_shifts(ctx, n):
     # sizes for every register layout (rows per thread 1 .. 16) and the
     # global-memory kernel above 8192; shifts off the spectrum, so no

@@ -4,10 +4,14 @@
 // not produce.  The reference ships the robust complex primitives but no
 // caller (robust.c:639-671 robust_cmplx_div, :677-767 solve_1x1_cmplx_system).
 //
-// One workgroup of 512 threads per shift, the whole vector on chip: thread t
-// holds rows t + 512 q, q < RPT (n <= 512 RPT <= 8192), in registers (y in
-// LDS for RPT = 16); above n = 8192 k_csolve_big keeps the vector in global
-// memory.  The sweep runs right to left over the columns of H - sigma I:
+// One workgroup per shift, the whole vector on chip: for n <= 512, 512
+// threads with one row each in registers; for 512 < n <= 8192, 1024 threads
+// (16 wavefronts: more latency hiding for the loads of H) where thread t
+// holds the running column of rows t + 1024 q, q < RPT <= 8, in registers
+// and the right-hand side y in LDS -- or, up to n = 2048 with more shifts
+// than CUs, 512 threads with RPT <= 4 rows each in registers (two
+// workgroups per CU).  Above n = 8192 k_csolve_big keeps the vector in
+// global memory.  The sweep runs right to left over the columns of H - sigma I:
 //   * the running column v (column st after the rotations so far) and the
 //     next column h = H(:, st-1) (- sigma on the diagonal) are combined by the
 //     unitary M = [[c, -conj(s)], [s, c]] that zeroes row st of the new column
@@ -43,10 +47,11 @@
 namespace hsrq {
 namespace {
 
-constexpr int kCThreads = 512;
+constexpr int kCThreads = 512;      // n <= 512 and the global-memory kernel
 constexpr int kCWaves = kCThreads / 64;
-constexpr int kCMaxRpt = 16;
-constexpr int kCMaxN = kCThreads * kCMaxRpt;
+constexpr int kCBigThreads = 1024;  // 512 < n <= 8192, y in LDS
+constexpr int kCMaxRpt = 8;
+constexpr int kCMaxN = kCBigThreads * kCMaxRpt;
 
 struct CArgs {
   int n;
@@ -87,7 +92,9 @@ __device__ __forceinline__ double norm3(double a, double b, double c) {
   return ldexp(sqrt((as * as + bs * bs) + cs * cs), e);
 }
 
-// Workgroup reductions (every thread of the workgroup calls them).
+// Workgroup reductions over W wavefronts (every thread of the workgroup
+// calls them).
+template <int W = kCWaves>
 __device__ __forceinline__ double block_max(double v, double* red) {
   v = wave_max(v);
   __syncthreads();
@@ -95,10 +102,11 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   __syncthreads();
   double r = red[0];
 #pragma unroll
-  for (int w = 1; w < kCWaves; ++w) r = fmax(r, red[w]);
+  for (int w = 1; w < W; ++w) r = fmax(r, red[w]);
   return r;
 }
 
+template <int W = kCWaves>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wave_sum(v);
   __syncthreads();
@@ -106,10 +114,11 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   __syncthreads();
   double r = red[0];
 #pragma unroll
-  for (int w = 1; w < kCWaves; ++w) r = r + red[w];
+  for (int w = 1; w < W; ++w) r = r + red[w];
   return r;
 }
 
+template <int W = kCWaves>
 __device__ __forceinline__ int block_min_int(int v, int* redi) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -118,24 +127,29 @@ __device__ __forceinline__ int block_min_int(int v, int* redi) {
   __syncthreads();
   int r = redi[0];
 #pragma unroll
-  for (int w = 1; w < kCWaves; ++w) r = min(r, redi[w]);
+  for (int w = 1; w < W; ++w) r = min(r, redi[w]);
   return r;
 }
 
-// YL: the right-hand side y lives in LDS instead of registers (the layout of
-// 16 rows per thread, whose registers would otherwise spill).
-template <int RPT, bool YL>
-__global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
-  __shared__ double ylr[YL ? kCMaxN : 1], yli[YL ? kCMaxN : 1];
+// NT threads per workgroup, RPT rows per thread; YL: the right-hand side y
+// lives in LDS instead of registers (NT = 1024: the 128 registers a thread
+// has at 4 wavefronts per SIMD hold the running column and the ring of H
+// columns only).
+constexpr int kCChunk = 512;   // rows per back-transform chunk
+template <int RPT, bool YL, int NT>
+__global__ __launch_bounds__(NT) void k_csolve(CArgs A) {
+  static_assert(YL || NT == kCChunk, "register-resident y: 512 threads");
+  constexpr int NW = NT / 64;
+  __shared__ double ylr[YL ? RPT * NT : 1], yli[YL ? RPT * NT : 1];
   struct Rec {   // one pivot step: rotation, quotient, bounds, protections
     double c, sr, si, qr, qi, by;
     int e1, slow, info;
   };
   __shared__ Rec rec[2];                 // by step parity
-  __shared__ double red[kCWaves];
-  __shared__ int redi[kCWaves];
-  __shared__ double stg[5][kCThreads];   // back-transform chunk: Re z, Im z, c, Re s, Im s
-  __shared__ double xo[2][kCThreads + 1];
+  __shared__ double red[NW];
+  __shared__ int redi[NW];
+  __shared__ double stg[5][kCChunk];   // back-transform chunk: Re z, Im z, c, Re s, Im s
+  __shared__ double xo[2][YL ? 1 : kCChunk + 1];
 
   const int k = A.idx ? A.idx[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
@@ -148,8 +162,8 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   double* __restrict__ SI = C + 2 * n;
 
   double hinf = 0.0;
-  for (int i = tid; i < n; i += kCThreads) hinf = fmax(hinf, A.rs[i]);
-  hinf = block_max(hinf, red);
+  for (int i = tid; i < n; i += NT) hinf = fmax(hinf, A.rs[i]);
+  hinf = block_max<NW>(hinf, red);
 
   const double eps = 0x1p-53;
   const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));
@@ -166,10 +180,12 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   // H columns ride a ring of D register slots: the column of step st is
   // loaded D steps before it is used (the step loop is unrolled by D so that
   // every slot index is a compile-time constant).
-  constexpr int D = RPT <= 2 ? 8 : (RPT == 4 ? 2 : (RPT == 8 ? 4 : 1));
+  // (depths measured on MI355X; at 8 rows per thread D = 1 keeps the
+  // spills of the 128-register budget of 1024 threads to a few)
+  constexpr int D = NT > kCChunk ? (RPT == 1 ? 4 : (RPT <= 4 ? 2 : 1)) : (RPT <= 2 ? 8 : 2);
   double vr[RPT], vi[RPT], yrr[YL ? 1 : RPT], yir[YL ? 1 : RPT];
-#define YR(q) (*(YL ? &ylr[tid + kCThreads * (q)] : &yrr[YL ? 0 : (q)]))
-#define YI(q) (*(YL ? &yli[tid + kCThreads * (q)] : &yir[YL ? 0 : (q)]))
+#define YR(q) (*(YL ? &ylr[tid + NT * (q)] : &yrr[YL ? 0 : (q)]))
+#define YI(q) (*(YL ? &yli[tid + NT * (q)] : &yir[YL ? 0 : (q)]))
   double ring[D][RPT], rsub[D];
   auto load_col = [&](int st, double (&h)[RPT], double& hs) {
     // column st-1 (rows 0 .. st-1 are used) and its subdiagonal entry
@@ -179,13 +195,13 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     // compiler wait for them early.
     const double* hcol = H + (int64_t)max(st - 1, 0) * ldH;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) h[q] = hcol[min(tid + kCThreads * q, n - 1)];
+    for (int q = 0; q < RPT; ++q) h[q] = hcol[min(tid + NT * q, n - 1)];
     const int sc = min(max(st, 1), n - 1);
     hs = H[n >= 2 ? sc + (int64_t)(sc - 1) * ldH : 0];
   };
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int row = tid + kCThreads * q;
+    const int row = tid + NT * q;
     const bool in = row < n;
     vr[q] = in ? H[row + (int64_t)(n - 1) * ldH] : 0.0;
     vi[q] = 0.0;
@@ -269,7 +285,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   };
   // value of register slot q_own of the thread holding row `row`, in every lane
   auto owner_value = [&](const double (&a)[RPT], int row) -> double {
-    const int qo = row / kCThreads;
+    const int qo = row / NT;
     double v = 0.0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) v = (q == qo) ? a[q] : v;
@@ -287,7 +303,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
 
   int E = 0, info = 0;
   double by = start_bound(n, A.its, eps3);
-  if (wave == ((n - 1) % kCThreads) >> 6)
+  if (wave == ((n - 1) % NT) >> 6)
     chain(n - 1, owner_value(vr, n - 1), owner_value(vi, n - 1), owner_y(yrr, ylr, n - 1),
           owner_y(yir, yli, n - 1), rsub[0], by);
   __syncthreads();
@@ -313,7 +329,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
       double a = 0.0;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
-        const int row = tid + kCThreads * q;
+        const int row = tid + NT * q;
         if (row <= st - 1) {
           const double hr = (row == st - 1) ? hc[q] - sa : hc[q];
           const double hi = (row == st - 1) ? -sb : 0.0;
@@ -322,7 +338,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
           a = fmax(a, nqs * (fabs(rr) + fabs(ri)) + ldexp(fabs(YR(q)) + fabs(YI(q)), -kk));
         }
       }
-      a = block_max(a, red);
+      a = block_max<NW>(a, red);
       int e2 = 0;
       if (a > ldexp(kOmega, -kk)) e2 = -(ilogb(a) + kk) - 1;
       if (e2 != 0) {
@@ -339,7 +355,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     }
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int row = tid + kCThreads * q;
+      const int row = tid + NT * q;
       if (row <= st - 1) {
         const double hr = (row == st - 1) ? hc[q] - sa : hc[q];
         const double hi = (row == st - 1) ? -sb : 0.0;
@@ -364,7 +380,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
       if (st >= 1) {
         step(st, ring[u]);
         load_col(st - D, ring[u], rsub[u]);   // the column of step st - D
-        if (wave == ((st - 1) % kCThreads) >> 6)
+        if (wave == ((st - 1) % NT) >> 6)
           chain(st - 1, owner_value(vr, st - 1), owner_value(vi, st - 1),
                 owner_y(yrr, ylr, st - 1), owner_y(yir, yli, st - 1), rsub[(u + 1) % D], by);
         __syncthreads();
@@ -377,7 +393,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     info |= R.info;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int row = tid + kCThreads * q;
+      const int row = tid + NT * q;
       if (row == 0) {
         YR(q) = R.qr;
         YI(q) = R.qi;
@@ -391,33 +407,33 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   __threadfence_block();
   __syncthreads();
 
-  // x = U_{n-1} ... U_1 z, chunk q = rows 512 q .. 512 q + 511 (slot q of
-  // every thread); xo[j] receives x of row 512 q + j - 1.
-  double tr = 0.0, ti = 0.0;   // the entering value (thread 0)
-#pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int g0 = kCThreads * q;
-    if (g0 < n) {
-      const int row = g0 + tid;
-      if (row < n) {
-        stg[0][tid] = YR(q);
-        stg[1][tid] = YI(q);
+  // x = U_{n-1} ... U_1 z by one thread, a sequential complex recurrence over
+  // chunks of 512 rows whose rotations are staged in LDS; x of row j - 1 is
+  // produced at element j.
+  if constexpr (YL) {
+    // z in LDS: x overwrites it in place (element j reads z_j, writes
+    // x_{j-1}, whose z was read at element j - 1)
+    double tr = 0.0, ti = 0.0;
+    for (int g0 = 0; g0 < n; g0 += kCChunk) {
+      const int cnt = min(kCChunk, n - g0);
+      if (tid < cnt) {
+        const int row = g0 + tid;
         stg[2][tid] = row >= 1 ? C[row] : 1.0;
         stg[3][tid] = row >= 1 ? SR[row] : 0.0;
         stg[4][tid] = row >= 1 ? SI[row] : 0.0;
       }
       __syncthreads();
       if (tid == 0) {
-        const int cnt = min(kCThreads, n - g0);
         for (int j = 0; j < cnt; ++j) {
-          const double zr = stg[0][j], zi = stg[1][j];
-          if (g0 + j == 0) {
+          const int row = g0 + j;
+          const double zr = ylr[row], zi = yli[row];
+          if (row == 0) {
             tr = zr;
             ti = zi;
           } else {
             const double cc = stg[2][j], s_r = stg[3][j], s_i = stg[4][j];
-            xo[0][j] = cc * tr - (s_r * zr + s_i * zi);
-            xo[1][j] = cc * ti - (s_r * zi - s_i * zr);
+            ylr[row - 1] = cc * tr - (s_r * zr + s_i * zi);
+            yli[row - 1] = cc * ti - (s_r * zi - s_i * zr);
             const double ntr = (s_r * tr - s_i * ti) + cc * zr;
             const double nti = (s_r * ti + s_i * tr) + cc * zi;
             tr = ntr;
@@ -426,35 +442,78 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
         }
       }
       __syncthreads();
-      if (tid + 1 < kCThreads && row + 1 < n) {   // produced at element tid + 1
-        YR(q) = xo[0][tid + 1];
-        YI(q) = xo[1][tid + 1];
-      }
-      if (q > 0 && tid == kCThreads - 1) {        // last row of the previous chunk
-        YR(q > 0 ? q - 1 : 0) = xo[0][0];
-        YI(q > 0 ? q - 1 : 0) = xo[1][0];
-      }
-      __syncthreads();
     }
-  }
-  if (tid == 0) {
-    xo[0][0] = tr;
-    xo[1][0] = ti;
-  }
-  __syncthreads();
+    if (tid == 0) {
+      ylr[n - 1] = tr;
+      yli[n - 1] = ti;
+    }
+    __syncthreads();
+  } else {
+    // z in registers: chunk q is slot q of every thread (NT = 512 rows),
+    // staged with its rotations; xo[j] receives x of row 512 q + j - 1
+    double tr = 0.0, ti = 0.0;   // the entering value (thread 0)
 #pragma unroll
-  for (int q = 0; q < RPT; ++q)
-    if (tid + kCThreads * q == n - 1) {
-      YR(q) = xo[0][0];
-      YI(q) = xo[1][0];
+    for (int q = 0; q < RPT; ++q) {
+      const int g0 = NT * q;
+      if (g0 < n) {
+        const int row = g0 + tid;
+        if (row < n) {
+          stg[0][tid] = YR(q);
+          stg[1][tid] = YI(q);
+          stg[2][tid] = row >= 1 ? C[row] : 1.0;
+          stg[3][tid] = row >= 1 ? SR[row] : 0.0;
+          stg[4][tid] = row >= 1 ? SI[row] : 0.0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          const int cnt = min(NT, n - g0);
+          for (int j = 0; j < cnt; ++j) {
+            const double zr = stg[0][j], zi = stg[1][j];
+            if (g0 + j == 0) {
+              tr = zr;
+              ti = zi;
+            } else {
+              const double cc = stg[2][j], s_r = stg[3][j], s_i = stg[4][j];
+              xo[0][j] = cc * tr - (s_r * zr + s_i * zi);
+              xo[1][j] = cc * ti - (s_r * zi - s_i * zr);
+              const double ntr = (s_r * tr - s_i * ti) + cc * zr;
+              const double nti = (s_r * ti + s_i * tr) + cc * zi;
+              tr = ntr;
+              ti = nti;
+            }
+          }
+        }
+        __syncthreads();
+        if (tid + 1 < NT && row + 1 < n) {   // produced at element tid + 1
+          YR(q) = xo[0][tid + 1];
+          YI(q) = xo[1][tid + 1];
+        }
+        if (q > 0 && tid == NT - 1) {        // last row of the previous chunk
+          YR(q > 0 ? q - 1 : 0) = xo[0][0];
+          YI(q > 0 ? q - 1 : 0) = xo[1][0];
+        }
+        __syncthreads();
+      }
     }
+    if (tid == 0) {
+      xo[0][0] = tr;
+      xo[1][0] = ti;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+      if (tid + NT * q == n - 1) {
+        YR(q) = xo[0][0];
+        YI(q) = xo[1][0];
+      }
+  }
 
   // ||x||_2, the first entry of largest modulus, normalisation
   double mx = 0.0;
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
-    if (tid + kCThreads * q < n) mx = fmax(mx, fmax(fabs(YR(q)), fabs(YI(q))));
-  mx = block_max(mx, red);
+    if (tid + NT * q < n) mx = fmax(mx, fmax(fabs(YR(q)), fabs(YI(q))));
+  mx = block_max<NW>(mx, red);
   double nrm = mx;
   int p = 0;
   if (mx > 0.0 && isfinite(mx)) {
@@ -463,7 +522,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
     int bi = INT_MAX;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
-      const int row = tid + kCThreads * q;
+      const int row = tid + NT * q;
       if (row < n) {
         const double a = ldexp(YR(q), -e), b = ldexp(YI(q), -e);
         const double t = a * a + b * b;
@@ -474,16 +533,16 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
         }
       }
     }
-    sum = block_sum(sum, red);
-    const double bmax = block_max(best, red);
-    p = block_min_int(best == bmax ? bi : INT_MAX, redi);
+    sum = block_sum<NW>(sum, red);
+    const double bmax = block_max<NW>(best, red);
+    p = block_min_int<NW>(best == bmax ? bi : INT_MAX, redi);
     nrm = ldexp(sqrt(sum), e);
   }
   const bool conv = isfinite(nrm) && nrm > ldexp(0.1 / sqrt((double)n), E);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < RPT; ++q)
-    if (tid + kCThreads * q == p) {
+    if (tid + NT * q == p) {
       xo[0][0] = YR(q);
       xo[1][0] = YI(q);
     }
@@ -499,7 +558,7 @@ __global__ __launch_bounds__(kCThreads) void k_csolve(CArgs A) {
   double* outi = A.vr + (int64_t)(2 * k + 1) * A.ldvr;
 #pragma unroll
   for (int q = 0; q < RPT; ++q) {
-    const int row = tid + kCThreads * q;
+    const int row = tid + NT * q;
     if (row < n) {
       const double a = YR(q), b = YI(q);
       outr[row] = scale ? (a * pr + b * pi) / nrm : a;
@@ -805,9 +864,9 @@ __global__ __launch_bounds__(kCThreads) void k_csolve_big(CArgs A, double* __res
   if (tid == 0) A.status[k] = (conv ? HSRQ_CONVERGED : 0) | (info ? HSRQ_PERTURBED : 0);
 }
 
-template <int RPT>
+template <int RPT, bool YL = false, int NT = kCThreads>
 hipError_t launch_csolve(const CArgs& a, int m, hipStream_t st) {
-  hipLaunchKernelGGL((k_csolve<RPT, RPT == kCMaxRpt>), dim3(m), dim3(kCThreads), 0, st, a);
+  hipLaunchKernelGGL((k_csolve<RPT, YL, NT>), dim3(m), dim3(NT), 0, st, a);
   return hipGetLastError();
 }
 
@@ -838,17 +897,26 @@ extern "C" int hsrq_solve_complex_device(hsrq_context* ctx, int n, const double*
   int* dst = reinterpret_cast<int*>(buf + nd * sizeof(double));
   int* didx = dst + m;
   CArgs a{n, dH, ldH, dwr, dwi, dvr, ldvr, dst, work, rs, -1, nullptr};
-  const int rpt = (n + kCThreads - 1) / kCThreads;
+  const int rpt = (n + kCBigThreads - 1) / kCBigThreads;   // rows per thread above 512
   auto pass = [&](int count) -> hipError_t {
     if (big) {
       hipLaunchKernelGGL(k_csolve_big, dim3(count), dim3(kCThreads), 0, ctx->stream, a, state);
       return hipGetLastError();
     }
-    if (rpt <= 1) return launch_csolve<1>(a, count, ctx->stream);
-    if (rpt <= 2) return launch_csolve<2>(a, count, ctx->stream);
-    if (rpt <= 4) return launch_csolve<4>(a, count, ctx->stream);
-    if (rpt <= 8) return launch_csolve<8>(a, count, ctx->stream);
-    return launch_csolve<16>(a, count, ctx->stream);
+    if (n <= kCThreads) return launch_csolve<1>(a, count, ctx->stream);
+    // Up to 2048 rows the 512-thread register layout fits two workgroups per
+    // CU (the 1024-thread one, one): with more shifts than CUs it finishes
+    // first, with at most one shift per CU the 1024-thread one does
+    // (measured on MI355X: n = 2000, 512 shifts 3.6 vs 5.3 ms; 256 shifts
+    // 2.67 vs 2.60 ms; n = 1000, 256 shifts 1.40 vs 1.17 ms).
+    if (rpt <= 2 && count > std::max(ctx->cus, 1)) {
+      if (n <= 2 * kCThreads) return launch_csolve<2>(a, count, ctx->stream);
+      return launch_csolve<4>(a, count, ctx->stream);
+    }
+    if (rpt <= 1) return launch_csolve<1, true, kCBigThreads>(a, count, ctx->stream);
+    if (rpt <= 2) return launch_csolve<2, true, kCBigThreads>(a, count, ctx->stream);
+    if (rpt <= 4) return launch_csolve<4, true, kCBigThreads>(a, count, ctx->stream);
+    return launch_csolve<8, true, kCBigThreads>(a, count, ctx->stream);
   };
   std::vector<int> st(m, 0), todo;
   hipError_t e = launch_rowsums(n, dH, ldH, part, rs, ctx->stream);

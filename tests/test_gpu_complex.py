@@ -68,9 +68,9 @@ def test_parity_complex_eigenvalues(ctx, n, seed):
 
 @pytest.mark.parametrize("n", [1, 1025, 2049, 3000, 4096, 4097, 5000, 8192, 8193, 10000])
 def test_parity_far_shifts(ctx, n):
-    # sizes for every register layout (rows per thread 1 .. 16) and the
-    # global-memory kernel above 8192; shifts off the spectrum, so no
-    # eigenvalue computation is needed
+    # sizes for every layout (512 threads with one row each; 1024 threads
+    # with 1, 2, 4, 8 rows each) and the global-memory kernel above 8192;
+    # shifts off the spectrum, so no eigenvalue computation is needed
     h = w.gaussian_hessenberg(n, n)
     rng = np.random.default_rng(n)
     sig = rng.uniform(-3, 3, 6) * np.sqrt(n) + 1j * rng.uniform(0.1, 2.0, 6) * np.sqrt(n)
@@ -79,6 +79,27 @@ def test_parity_far_shifts(ctx, n):
     assert np.isfinite(x).all()
     assert np.array_equal(st, sto)
     assert np.abs(x - xo).max() <= TOL
+
+
+@pytest.mark.parametrize("n", [700, 1500])
+def test_parity_more_shifts_than_cus(ctx, n):
+    # more shifts than compute units: up to n = 2048 the 512-thread register
+    # layout (two workgroups per CU) runs instead of the 1024-thread one
+    m = torch.cuda.get_device_properties(0).multi_processor_count + 40
+    h = w.gaussian_hessenberg(n, n + 1)
+    rng = np.random.default_rng(n)
+    sig = rng.uniform(-3, 3, m) * np.sqrt(n) + 1j * rng.uniform(0.1, 2.0, m) * np.sqrt(n)
+    x, st = gpu_complex(ctx, h, sig)
+    xo, sto = oracle.complex_inverse_iteration(h, sig.real, sig.imag)
+    assert np.isfinite(x).all()
+    assert np.array_equal(st, sto)
+    assert np.abs(x - xo).max() <= TOL
+    # the same shifts a few at a time (the 1024-thread layout): the sweeps are
+    # the same arithmetic, only the norm's sum is grouped over 16 wavefronts
+    # instead of 8
+    x8, st8 = gpu_complex(ctx, h, sig[:8])
+    assert np.array_equal(st8, st[:8])
+    assert np.abs(x8 - x[:, :8]).max() <= 1e-14
 
 
 def test_conjugate_shifts_conjugate_vectors(ctx):

@@ -125,8 +125,9 @@ def test_real_shifts_match_real_path(ctx):
     assert (1.0 - c).max() <= TOL
 
 
-def test_graded(ctx):
-    n = 400
+@pytest.mark.parametrize("n", [400, 1500])
+def test_graded(ctx, n):
+    # both kinds of layout: 512 threads (n <= 512), 1024 threads with y in LDS
     h0 = w.gaussian_hessenberg(n, 31)
     ev = np.linalg.eigvals(h0)
     sig = ev[ev.imag > 0][:12]
@@ -152,9 +153,10 @@ def test_rejects_and_empty(ctx):
                                        H.data_ptr(), H.data_ptr(), 1, -1, None) == -1
 
 
-def test_leading_dimensions(ctx):
+@pytest.mark.parametrize("n", [300, 1300])
+def test_leading_dimensions(ctx, n):
     # ldH > n and ldvr > n: the padding is neither read as data nor written
-    n, ld, ldv = 300, 307, 311
+    ld, ldv = n + 7, n + 11
     h, sig = eig_case(n, 4 + 300, 8)
     x, st = gpu_complex(ctx, h, sig)
     Ht = torch.full((n, ld), np.nan, dtype=torch.float64, device="cuda")

@@ -329,10 +329,11 @@ __device__ __forceinline__ double pick_tree(const double* a, int q, int bit) {
     return (q & bit) ? hi : lo;
   }
 }
-template <int RPL>
+// a[q] of the first NS slots (NS = RPL by default)
+template <int RPL, int NS = RPL>
 __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
-  static_assert((RPL & (RPL - 1)) == 0, "power of two");
-  return pick_tree<RPL>(a, q, RPL / 2);
+  static_assert((NS & (NS - 1)) == 0 && NS <= RPL, "power of two");
+  return pick_tree<NS>(a, q, NS / 2);
 }
 
 template <int RPL>
@@ -882,10 +883,13 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   // Every element gets the operations of k_diag in the same order; with the
   // rare cases decided before any vector work, no step needs copies of v, y
   // or R to fall back on (registers for 3 wavefronts per SIMD).
-  auto vscal = [&](auto Lc, int st, const double* hc, bool full, VOut& o)
+  // NSc: the row slots that can hold active rows (the first NS; NS = RPL / 2
+  // once the sweep is past the middle of the tile, s < 8 RPL).
+  auto vscal = [&](auto Lc, auto NSc, int st, const double* hc, bool full, VOut& o)
       __attribute__((always_inline)) -> bool {
     constexpr int L = decltype(Lc)::value;
-    const double vs = bc16<L>(pick<RPL>(v, st >> 4), st & 15);
+    constexpr int NS = decltype(NSc)::value;
+    const double vs = bc16<L>(pick<RPL, NS>(v, st >> 4), st & 15);
     const double hsub = hc[st];
     const double hdd = hc[st - 1] - mu;
     double c, s, r, rinv;
@@ -915,10 +919,13 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     o.hdd = hdd;
     return rare;
   };
-  auto vvec = [&](int st, const double* hc, const VOut& o, double (&nr)[RPL])
+  // slots from NS on hold finished rows: v kept, nr already 0 (the step
+  // before set it so: its active rows end below 16 NS)
+  auto vvec = [&](auto NSc, int st, const double* hc, const VOut& o, double (&nr)[RPL])
       __attribute__((always_inline)) {
+    constexpr int NS = decltype(NSc)::value;
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
+    for (int q = 0; q < NS; ++q) {
       const int row = t + 16 * q;
       const bool act = row <= st - 1;
       const double h = (row == st - 1) ? o.hdd : hc[row];
@@ -927,10 +934,11 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
       v[q] = act ? fma(o.c, h, o.s * vq) : vq;
     }
   };
-  auto yscal = [&](auto Lc, int st, const VOut& o, double& yq, double& byn)
+  auto yscal = [&](auto Lc, auto NSc, int st, const VOut& o, double& yq, double& byn)
       __attribute__((always_inline)) -> bool {
     constexpr int L = decltype(Lc)::value;
-    const double yj = bc16<L>(pick<RPL>(y, st >> 4), st & 15);
+    constexpr int NS = decltype(NSc)::value;
+    const double yj = bc16<L>(pick<RPL, NS>(y, st >> 4), st & 15);
     const double acsr = fabs(o.r);
     bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
     yq = yj * o.rinv;
@@ -940,9 +948,11 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   };
   // r is 0 on the rows outside 0 .. st-1 (vvec), where the update leaves y
   // as it is (yq is finite on this path): one select per slot, for row st
-  auto yvec = [&](int st, double yq, const double (&r)[RPL]) __attribute__((always_inline)) {
+  auto yvec = [&](auto NSc, int st, double yq, const double (&r)[RPL])
+      __attribute__((always_inline)) {
+    constexpr int NS = decltype(NSc)::value;
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
+    for (int q = 0; q < NS; ++q) {
       const int row = t + 16 * q;
       const double u = fma(-yq, r[q], y[q]);
       y[q] = (row == st) ? yq : u;
@@ -1001,25 +1011,26 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     double rr[RPL];   // column of R of the pending y step
     {
       const double* hc = column(nb - 2);
-      if (vscal(IC<-1>{}, nb - 1, hc, false, cv)) vscal(IC<-1>{}, nb - 1, hc, true, cv);
-      vvec(nb - 1, hc, cv, rr);
+      if (vscal(IC<-1>{}, IC<RPL>{}, nb - 1, hc, false, cv))
+        vscal(IC<-1>{}, IC<RPL>{}, nb - 1, hc, true, cv);
+      vvec(IC<RPL>{}, nb - 1, hc, cv, rr);
       Bv = cv.bv;
       keep_rotation(nb - 1, cv);
     }
     // Fast step s (y-step s with v-step s-1 interleaved), compile-time lane
     // LY of row s: commits and returns true unless a shift of this wave needs
     // a rare case, in which case nothing changed and the general step runs.
-    auto fast_step = [&](auto LYc, int s) __attribute__((always_inline)) -> bool {
+    auto fast_step = [&](auto LYc, auto NSc, int s) __attribute__((always_inline)) -> bool {
       constexpr int LY = decltype(LYc)::value;
       constexpr int LV = (LY + 15) & 15;
       const double* hc = column(s - 2);
       VOut nvo;
       double byn, yq;
-      const bool rv = vscal(IC<LV>{}, s - 1, hc, false, nvo);
-      const bool ry = yscal(IC<LY>{}, s, cv, yq, byn);
+      const bool rv = vscal(IC<LV>{}, NSc, s - 1, hc, false, nvo);
+      const bool ry = yscal(IC<LY>{}, NSc, s, cv, yq, byn);
       if (__any(rv | ry | exact)) return false;   // nothing changed: the general step runs
-      yvec(s, yq, rr);
-      vvec(s - 1, hc, nvo, rr);   // R column of step s-1 replaces that of step s
+      yvec(NSc, s, yq, rr);
+      vvec(NSc, s - 1, hc, nvo, rr);   // R column of step s-1 replaces that of step s
       By = byn;
       Bv = nvo.bv;
       cv = nvo;
@@ -1031,53 +1042,63 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
       const double* hc = column(s - 2);
       VOut nvo;
       double byn, yq;
-      if (vscal(IC<-1>{}, s - 1, hc, false, nvo)) vscal(IC<-1>{}, s - 1, hc, true, nvo);
-      if (yscal(IC<-1>{}, s, cv, yq, byn) || exact) {
+      if (vscal(IC<-1>{}, IC<RPL>{}, s - 1, hc, false, nvo))
+        vscal(IC<-1>{}, IC<RPL>{}, s - 1, hc, true, nvo);
+      if (yscal(IC<-1>{}, IC<RPL>{}, s, cv, yq, byn) || exact) {
         ystep_slow(s, cv, rr);
       } else {
-        yvec(s, yq, rr);
+        yvec(IC<RPL>{}, s, yq, rr);
         By = byn;
       }
-      vvec(s - 1, hc, nvo, rr);
+      vvec(IC<RPL>{}, s - 1, hc, nvo, rr);
       Bv = nvo.bv;
       cv = nvo;
       keep_rotation(s - 1, cv);
     };
     int s = nb - 1;
+    // 16 fast steps s = 16 B + 15 .. 16 B over the first NS row slots; false
+    // at the first that fails (s is then that step)
+    auto block = [&](auto NSc) __attribute__((always_inline)) -> bool {
+      if (!fast_step(IC<15>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<14>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<13>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<12>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<11>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<10>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<9>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<8>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<7>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<6>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<5>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<4>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<3>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<2>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<1>{}, NSc, s)) return false;
+      --s;
+      if (!fast_step(IC<0>{}, NSc, s)) return false;
+      --s;
+      return true;
+    };
     while (s >= 2) {
       if ((s & 15) == 15 && s >= 17) {
-        if (!fast_step(IC<15>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<14>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<13>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<12>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<11>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<10>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<9>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<8>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<7>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<6>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<5>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<4>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<3>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<2>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<1>{}, s)) goto general;
-        --s;
-        if (!fast_step(IC<0>{}, s)) goto general;
-        --s;
+        // past the middle of the tile the upper half of the slots is done:
+        // a second copy of the block updates the lower half only
+        const bool ok = s >= 8 * RPL ? block(IC<RPL>{}) : block(IC<RPL / 2>{});
+        if (!ok) goto general;
         continue;
       }
     general:
@@ -1086,10 +1107,10 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     }
     {
       double byn, yq;
-      if (yscal(IC<1>{}, 1, cv, yq, byn) || exact) {
+      if (yscal(IC<1>{}, IC<RPL>{}, 1, cv, yq, byn) || exact) {
         ystep_slow(1, cv, rr);
       } else {
-        yvec(1, yq, rr);
+        yvec(IC<RPL>{}, 1, yq, rr);
         By = byn;
       }
     }

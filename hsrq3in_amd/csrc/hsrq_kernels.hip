@@ -329,6 +329,10 @@ __device__ __forceinline__ double pick_tree(const double* a, int q, int bit) {
     return (q & bit) ? hi : lo;
   }
 }
+// compile-time int argument of the generic lambdas below
+template <int V>
+using IC = std::integral_constant<int, V>;
+
 // a[q] of the first NS slots (NS = RPL by default)
 template <int RPL, int NS = RPL>
 __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
@@ -447,9 +451,12 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   struct VOut {
     double c, s, r, rinv, br, bv;
   };
-  auto vstep = [&](int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
+  // NSc: the row slots that can hold active rows (the first NS; see the
+  // step loop); nr and nv are written for those only
+  auto vstep = [&](auto NSc, int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
                    double (&nv)[RPL]) -> bool {
-    const double vs = readlane(pick<RPL>(v, st >> 6), st & 63);
+    constexpr int NS = decltype(NSc)::value;
+    const double vs = readlane(pick<RPL, NS>(v, st >> 6), st & 63);
     const double hsub = hc[st];
     const double hdd = hc[st - 1] - mu;
     double c, s, r, rinv;
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     o.br = fma(fabs(c), Bv, fabs(s) * Hm) * slack;
     o.bv = fma(fabs(c), Hm, fabs(s) * Bv) * slack;
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
+    for (int q = 0; q < NS; ++q) {
       const int row = lane + 64 * q;
       const bool act = row <= st - 1;
       const double h = (row == st - 1) ? hdd : hc[row];
@@ -491,16 +498,17 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   // y(0:st-1) -= y(st) R(0:st-1, st) when no protection can trigger
   // (|R(st,st)| >= smin, the quotient and the update provably below Omega
   // from the running bounds).
-  auto ystep_fast = [&](int st, const VOut& o, const double (&r)[RPL], double (&ny)[RPL],
-                        double& byn) -> bool {
-    const double yj = readlane(pick<RPL>(y, st >> 6), st & 63);
+  auto ystep_fast = [&](auto NSc, int st, const VOut& o, const double (&r)[RPL],
+                        double (&ny)[RPL], double& byn) -> bool {
+    constexpr int NS = decltype(NSc)::value;
+    const double yj = readlane(pick<RPL, NS>(y, st >> 6), st & 63);
     const double acsr = fabs(o.r);
     bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
     const double yq = yj * o.rinv;
     byn = fma(fabs(yq), o.br, By) * slack;
     rare = rare | !(byn <= kOmega);
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
+    for (int q = 0; q < NS; ++q) {
       // r is 0 on the rows outside 0 .. st-1 (vstep), where the update
       // leaves y as it is (yq is finite when this result is used)
       const int row = lane + 64 * q;
@@ -557,10 +565,11 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     By = ldexp(a, kk + e2) * slack;
   };
 
-  auto keep_rotation = [&](int st, const VOut& o) {
+  auto keep_rotation = [&](auto NSc, int st, const VOut& o) {
+    constexpr int NS = decltype(NSc)::value;
     const int own = (lane == (st & 63)) ? (st >> 6) : -1;
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) {
+    for (int q = 0; q < NS; ++q) {
       cr[q] = (own == q) ? o.c : cr[q];
       sr[q] = (own == q) ? o.s : sr[q];
     }
@@ -572,42 +581,57 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     double rr[RPL], nr[RPL], nv[RPL], ny[RPL];
     {
       const double* hc = column(nb - 2);
-      if (vstep(nb - 1, hc, false, cv, nr, nv)) vstep(nb - 1, hc, true, cv, nr, nv);
+      if (vstep(IC<RPL>{}, nb - 1, hc, false, cv, nr, nv))
+        vstep(IC<RPL>{}, nb - 1, hc, true, cv, nr, nv);
 #pragma unroll
       for (int q = 0; q < RPL; ++q) {
         v[q] = nv[q];
         rr[q] = nr[q];
       }
       Bv = cv.bv;
-      keep_rotation(nb - 1, cv);
+      keep_rotation(IC<RPL>{}, nb - 1, cv);
     }
     // Steady state: v-step s-1 and y-step s are independent and interleave.
-    for (int s = nb - 1; s >= 2; --s) {
-      const double* hc = column(s - 2);
-      VOut nvo;
-      double byn;
-      const bool rv = vstep(s - 1, hc, false, nvo, nr, nv);
-      const bool ry = ystep_fast(s, cv, rr, ny, byn);
-      if (rv) vstep(s - 1, hc, true, nvo, nr, nv);
-      if (ry || exact) {
-        ystep_slow(s, cv, rr);
-      } else {
+    // Steps s hi .. lo over the first NS row slots: the slots from NS on hold
+    // finished rows only (v and y kept, R column rr already 0 there).
+    auto steps = [&](auto NSc, int hi, int lo) {
+      constexpr int NS = decltype(NSc)::value;
+      for (int s = hi; s >= lo; --s) {
+        const double* hc = column(s - 2);
+        VOut nvo;
+        double byn;
+        const bool rv = vstep(NSc, s - 1, hc, false, nvo, nr, nv);
+        const bool ry = ystep_fast(NSc, s, cv, rr, ny, byn);
+        if (rv) vstep(NSc, s - 1, hc, true, nvo, nr, nv);
+        if (ry || exact) {
+          ystep_slow(s, cv, rr);
+        } else {
 #pragma unroll
-        for (int q = 0; q < RPL; ++q) y[q] = ny[q];
-        By = byn;
-      }
+          for (int q = 0; q < NS; ++q) y[q] = ny[q];
+          By = byn;
+        }
 #pragma unroll
-      for (int q = 0; q < RPL; ++q) {
-        v[q] = nv[q];
-        rr[q] = nr[q];
+        for (int q = 0; q < NS; ++q) {
+          v[q] = nv[q];
+          rr[q] = nr[q];
+        }
+        Bv = nvo.bv;
+        cv = nvo;
+        keep_rotation(NSc, s - 1, cv);
       }
-      Bv = nvo.bv;
-      cv = nvo;
-      keep_rotation(s - 1, cv);
+    };
+    if constexpr (RPL >= 2) {
+      // below s = 32 RPL (y-step s, v-step s - 1) every active row and row s
+      // lie in the lower half of the slots
+      constexpr int MID = 32 * RPL;
+      steps(IC<RPL>{}, nb - 1, MID);
+      steps(IC<RPL / 2>{}, min(nb - 1, MID - 1), 2);
+    } else {
+      steps(IC<RPL>{}, nb - 1, 2);
     }
     {
       double byn;
-      if (ystep_fast(1, cv, rr, ny, byn) || exact) {
+      if (ystep_fast(IC<RPL>{}, 1, cv, rr, ny, byn) || exact) {
         ystep_slow(1, cv, rr);
       } else {
 #pragma unroll
@@ -769,8 +793,6 @@ __device__ __forceinline__ double bc16(double x, int lrt) {
   else
     return bcast16<L>(x);
 }
-template <int V>
-using IC = std::integral_constant<int, V>;
 
 // Three wavefronts per SIMD (at most 168 VGPRs) where the rows per lane allow
 // it: the packed sweeps run where many shifts are in flight, and their rate

@@ -620,7 +620,13 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
         keep_rotation(NSc, s - 1, cv);
       }
     };
-    if constexpr (RPL >= 2) {
+    if constexpr (RPL >= 4) {
+      // below s = 64 NS (y-step s, v-step s - 1) every active row and row s
+      // lie in the first NS slots: all, half, then a quarter of them
+      steps(IC<RPL>{}, nb - 1, 32 * RPL);
+      steps(IC<RPL / 2>{}, min(nb - 1, 32 * RPL - 1), 16 * RPL);
+      steps(IC<RPL / 4>{}, min(nb - 1, 16 * RPL - 1), 2);
+    } else if constexpr (RPL >= 2) {
       // below s = 32 RPL (y-step s, v-step s - 1) every active row and row s
       // lie in the lower half of the slots
       constexpr int MID = 32 * RPL;

@@ -455,8 +455,10 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   // step loop); nr and nv are written for those only
   auto vstep = [&](auto NSc, int st, const double* hc, bool full, VOut& o, double (&nr)[RPL],
                    double (&nv)[RPL]) -> bool {
-    constexpr int NS = decltype(NSc)::value;
-    const double vs = readlane(pick<RPL, NS>(v, st >> 6), st & 63);
+    constexpr int NS = decltype(NSc)::value % 100;
+    constexpr bool A0 = decltype(NSc)::value >= 100;   // slot 0: all rows active, none st-1
+    // (with A0, row st lies in the last of the NS slots)
+    const double vs = readlane(A0 ? v[NS - 1] : pick<RPL, NS>(v, st >> 6), st & 63);
     const double hsub = hc[st];
     const double hdd = hc[st - 1] - mu;
     double c, s, r, rinv;
@@ -486,10 +488,16 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
       const int row = lane + 64 * q;
-      const bool act = row <= st - 1;
-      const double h = (row == st - 1) ? hdd : hc[row];
-      nr[q] = act ? fma(c, v[q], -(s * h)) : 0.0;
-      nv[q] = act ? fma(c, h, s * v[q]) : v[q];
+      if (A0 && q == 0) {
+        const double h = hc[row];
+        nr[q] = fma(c, v[q], -(s * h));
+        nv[q] = fma(c, h, s * v[q]);
+      } else {
+        const bool act = row <= st - 1;
+        const double h = (row == st - 1) ? hdd : hc[row];
+        nr[q] = act ? fma(c, v[q], -(s * h)) : 0.0;
+        nv[q] = act ? fma(c, h, s * v[q]) : v[q];
+      }
     }
     return rare;
   };
@@ -500,8 +508,9 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   // from the running bounds).
   auto ystep_fast = [&](auto NSc, int st, const VOut& o, const double (&r)[RPL],
                         double (&ny)[RPL], double& byn) -> bool {
-    constexpr int NS = decltype(NSc)::value;
-    const double yj = readlane(pick<RPL, NS>(y, st >> 6), st & 63);
+    constexpr int NS = decltype(NSc)::value % 100;
+    constexpr bool A0 = decltype(NSc)::value >= 100;
+    const double yj = readlane(A0 ? y[NS - 1] : pick<RPL, NS>(y, st >> 6), st & 63);
     const double acsr = fabs(o.r);
     bool rare = !(acsr >= smin) | ((acsr < 1.0) & (fabs(yj) > acsr * kOmega));
     const double yq = yj * o.rinv;
@@ -513,7 +522,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       // leaves y as it is (yq is finite when this result is used)
       const int row = lane + 64 * q;
       const double u = fma(-yq, r[q], y[q]);
-      ny[q] = (row == st) ? yq : u;
+      ny[q] = (A0 && q == 0) ? u : ((row == st) ? yq : u);
     }
     return rare;
   };
@@ -566,10 +575,11 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   };
 
   auto keep_rotation = [&](auto NSc, int st, const VOut& o) {
-    constexpr int NS = decltype(NSc)::value;
+    constexpr int NS = decltype(NSc)::value % 100;
+    constexpr int Q0 = decltype(NSc)::value >= 100 ? 1 : 0;   // slot 0 owns no row st
     const int own = (lane == (st & 63)) ? (st >> 6) : -1;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
+    for (int q = Q0; q < NS; ++q) {
       cr[q] = (own == q) ? o.c : cr[q];
       sr[q] = (own == q) ? o.s : sr[q];
     }
@@ -595,7 +605,7 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
     // Steps s hi .. lo over the first NS row slots: the slots from NS on hold
     // finished rows only (v and y kept, R column rr already 0 there).
     auto steps = [&](auto NSc, int hi, int lo) {
-      constexpr int NS = decltype(NSc)::value;
+      constexpr int NS = decltype(NSc)::value % 100;
       for (int s = hi; s >= lo; --s) {
         const double* hc = column(s - 2);
         VOut nvo;
@@ -626,6 +636,13 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
       steps(IC<RPL>{}, nb - 1, 32 * RPL);
       steps(IC<RPL / 2>{}, min(nb - 1, 32 * RPL - 1), 16 * RPL);
       steps(IC<RPL / 4>{}, min(nb - 1, 16 * RPL - 1), 2);
+    } else if constexpr (RPL == 2) {
+      // from s = 66 on, slot 0 (rows 0 .. 63) holds active rows only, none
+      // of them row s or s - 2 (IC<100 + NS>: that slot without predicates);
+      // below s = 64 every active row and row s lie in slot 0
+      steps(IC<100 + RPL>{}, nb - 1, 66);
+      steps(IC<RPL>{}, min(nb - 1, 65), 64);
+      steps(IC<RPL / 2>{}, min(nb - 1, 63), 2);
     } else if constexpr (RPL >= 2) {
       // below s = 32 RPL (y-step s, v-step s - 1) every active row and row s
       // lie in the lower half of the slots

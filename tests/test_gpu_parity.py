@@ -46,9 +46,13 @@ def test_selftest(ctx):
     ctx.selftest()
 
 
+# (321, 322, 323 at tile 128; 641, 642 at tile 256: the ragged first tile
+# starts its sweep on the boundaries where the step loop changes the row
+# slots it updates, s = 64, 65, 66 and 128, 129)
 @pytest.mark.parametrize("n,tile", [(64, 64), (65, 64), (130, 64), (200, 64), (333, 128),
                                     (700, 64), (700, 128), (700, 192), (700, 256),
-                                    (1000, 128)])
+                                    (1000, 128), (321, 128), (322, 128), (323, 128),
+                                    (641, 256), (642, 256)])
 def test_parity_gaussian(ctx, n, tile):
     H, wr = case(n, seed=n + tile, m=24)
     x, wo, st, res = gpu_solve(ctx, H, wr, tile)

@@ -181,8 +181,54 @@ def colmajor_to_host(t) -> np.ndarray:
     return np.asfortranarray(t.detach().cpu().numpy().T)
 
 
+def _check_matrix(t, name: str, n: int | None = None):
+    """A device matrix in the transposed-storage layout: contiguous float64
+    CUDA tensor (rows, ld) with ld >= rows (rows == n when n is given)."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda \
+            or t.dim() != 2 or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous 2-D float64 CUDA tensor")
+    if n is not None and t.shape[0] != n:
+        raise ValueError(f"{name} has {t.shape[0]} rows, expected {n}")
+
+
+def _check_problem(Ht, wrt, vrt):
+    """Shapes and types of one device problem (Ht (n, ldH >= n), wrt (m,),
+    vrt (m, ldvr >= n)); returns (n, m)."""
+    import torch
+    _check_matrix(Ht, "Ht")
+    n = Ht.shape[0]
+    if Ht.shape[1] < n:
+        raise ValueError("Ht must be (n, ldH) with ldH >= n")
+    if not isinstance(wrt, torch.Tensor) or wrt.dtype != torch.float64 or not wrt.is_cuda \
+            or not wrt.is_contiguous():
+        raise ValueError("wrt must be a contiguous float64 CUDA tensor")
+    m = wrt.numel()
+    _check_matrix(vrt, "vrt", m)
+    if m and vrt.shape[1] < n:
+        raise ValueError("vrt must be (m, ldvr) with ldvr >= n")
+    if not (Ht.device == wrt.device == vrt.device):
+        raise ValueError("Ht, wrt and vrt must be on one device")
+    return n, m
+
+
+def _after_torch(stream_ptr: int, device):
+    """Orders the library stream after the work torch has queued on its
+    current stream (the library's streams are non-blocking: they do not wait
+    for the fills and copies that produced the inputs by themselves)."""
+    import torch
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(device))
+    torch.cuda.ExternalStream(stream_ptr, device=device).wait_event(ev)
+
+
 class Context:
-    """Device, stream and settings of libhsrq (hsrq_context)."""
+    """Device, stream and settings of libhsrq (hsrq_context).
+
+    Every method that hands device tensors to the library first orders the
+    context stream after torch's current stream, so inputs produced by torch
+    are complete.  The synchronous methods return with the results in place;
+    Plan.execute is asynchronous (Plan.sync)."""
 
     def __init__(self, device: int = 0, tile: int = 0, exact_norms: bool = False,
                  packing: int = 0, precision: int = 0):
@@ -250,9 +296,9 @@ class Context:
         """Solve on device tensors (column-major via transposed storage):
         Ht (n, ldH) with Ht[j, i] = H(i, j); wrt (m,); vrt (m, ldvr).
         Returns the per-shift status array."""
-        n = Ht.shape[0]
-        m = wrt.numel()
+        n, m = _check_problem(Ht, wrt, vrt)
         st = np.zeros(max(m, 1), dtype=np.int32)
+        _after_torch(self.stream_ptr(), Ht.device)
         rc = lib().hsrq_solve_device(self._h, n, Ht.data_ptr(), Ht.shape[1], m, wrt.data_ptr(),
                                      vrt.data_ptr(), vrt.shape[1], tlsz,
                                      st.ctypes.data_as(_ip))
@@ -263,6 +309,7 @@ class Context:
     def _problems(items):
         arr = (Problem * len(items))()
         for i, (Ht, wrt, vrt) in enumerate(items):
+            _check_problem(Ht, wrt, vrt)
             arr[i] = Problem(Ht.shape[0], Ht.data_ptr(), Ht.shape[1], wrt.numel(), wrt.data_ptr(),
                              vrt.data_ptr(), vrt.shape[1])
         return arr
@@ -272,6 +319,8 @@ class Context:
         arr = self._problems(items)
         total = sum(w.numel() for _, w, _ in items)
         st = np.zeros(max(total, 1), dtype=np.int32)
+        if items:
+            _after_torch(self.stream_ptr(), items[0][0].device)
         _check(lib().hsrq_solve_batch_device(self._h, len(items), arr, tlsz,
                                              st.ctypes.data_as(_ip)), "hsrq_solve_batch_device")
         out, pos = [], 0
@@ -286,6 +335,8 @@ class Context:
         arr = self._problems(items)
         t, pk, ln = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ms = ctypes.c_double()
+        if items:
+            _after_torch(self.stream_ptr(), items[0][0].device)
         _check(lib().hsrq_autotune(self._h, len(items), arr, ctypes.byref(t), ctypes.byref(pk),
                                    ctypes.byref(ln), ctypes.byref(ms)), "hsrq_autotune")
         self.packing = pk.value
@@ -297,8 +348,9 @@ class Context:
 
     def residuals(self, Ht, wrt, vrt) -> np.ndarray:
         """||(H - wr_k I) x_k|| / (||H||_oo ||x_k|| 2^-52) per eigenvector (device)."""
-        m = wrt.numel()
+        _, m = _check_problem(Ht, wrt, vrt)
         res = np.zeros(max(m, 1), dtype=np.float64)
+        _after_torch(self.stream_ptr(), Ht.device)
         _check(lib().hsrq_residuals_device(self._h, Ht.shape[0], Ht.data_ptr(), Ht.shape[1], m,
                                            wrt.data_ptr(), vrt.data_ptr(), vrt.shape[1],
                                            res.ctypes.data_as(_dp)), "hsrq_residuals_device")
@@ -311,9 +363,16 @@ class Context:
         / imaginary parts of x_k (hsrq_solve_complex_device); unconverged shifts
         are retried with dlaein's restart vectors up to max_restarts times."""
         import torch
+        _check_matrix(Ht, "Ht")
         n, m = Ht.shape[0], wrt.numel()
         if vrt is None:
             vrt = torch.zeros((2 * m, n), dtype=torch.float64, device=Ht.device)
+        for t, name in ((wrt, "wrt"), (wit, "wit")):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.numel() != m:
+                raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of m entries")
+        _check_matrix(vrt, "vrt", 2 * m)
+        if Ht.shape[1] < n or vrt.shape[1] < n:
+            raise ValueError("Ht and vrt need leading dimensions >= n")
         st = np.zeros(max(m, 1), dtype=np.int32)
         torch.cuda.synchronize(Ht.device)
         _check(lib().hsrq_solve_complex_device(self._h, n, Ht.data_ptr(), Ht.shape[1], m,
@@ -367,6 +426,7 @@ class Context:
         Ht untouched) on overflow or, if exact, on a result that rounds."""
         import torch
         k = np.ascontiguousarray(k, dtype=np.int32)
+        _check_matrix(Ht, "Ht")
         n = Ht.shape[0]
         if k.size != n:
             raise ValueError("k needs n exponents")
@@ -383,9 +443,14 @@ class Context:
         with Zt[j, i] = Z(i, j); Xt (m, ldX) holds the eigenvectors of H as rows
         (like the solve's vrt); returns Vt (m, n) laid out the same way."""
         import torch
+        _check_matrix(Zt, "Zt")
+        _check_matrix(Xt, "Xt")
         n, m = Zt.shape[0], Xt.shape[0]
         if Vt is None:
             Vt = torch.empty((m, n), dtype=torch.float64, device=Xt.device)
+        _check_matrix(Vt, "Vt", m)
+        if min(Zt.shape[1], Xt.shape[1], Vt.shape[1]) < n:
+            raise ValueError("Zt, Xt and Vt need leading dimensions >= n")
         torch.cuda.synchronize(Xt.device)
         _check(lib().hsrq_backtransform_device(self._h, n, m, Zt.data_ptr(), Zt.shape[1],
                                                Xt.data_ptr(), Xt.shape[1], Vt.data_ptr(),
@@ -407,6 +472,10 @@ class Plan:
         self.total_m = sum(w.numel() for _, w, _ in items)
 
     def execute(self, graph: bool = False):
+        """Asynchronous on the context stream, after torch's current stream's
+        queued work; sync() waits for it."""
+        if self.items:
+            _after_torch(self.ctx.stream_ptr(), self.items[0][0].device)
         f = lib().hsrq_plan_execute_graph if graph else lib().hsrq_plan_execute
         _check(f(self._h), "hsrq_plan_execute")
 

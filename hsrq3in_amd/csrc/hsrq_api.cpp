@@ -1062,7 +1062,11 @@ int hsrq_residuals_device(hsrq_context* ctx, int n, const double* dH, int ldH, i
 }
 
 // Drop-in entry point (reference inverse_iteration.c:128-335): host
-// buffers in and out, synchronous, on the current device.
+// buffers in and out, synchronous, on the current device.  The device copies
+// of H, wr and vr are kept between calls (grown when a call needs more), so a
+// repeated call pays for the copies and the solve only.  H is copied as an
+// n x n block with the caller's pitch ldH: nothing past element
+// (n-1) ldH + n-1 is read, as in the reference.
 void tiled_inverse_iteration(int n, const double* H, int ldH, int nrhs, double* wr, int tlsz,
                              int rhs_tlsz, double* vr, int ldvr) {
   if (n <= 0 || !H || ldH < n || nrhs < 0 || tlsz <= 0 || tlsz % 4 != 0 || rhs_tlsz <= 0 ||
@@ -1073,27 +1077,38 @@ void tiled_inverse_iteration(int n, const double* H, int ldH, int nrhs, double* 
   if (nrhs == 0) return;
   static std::mutex mu;
   static hsrq_context* ctx = nullptr;
+  static double *dH = nullptr, *dwr = nullptr, *dvr = nullptr;
+  static size_t capH = 0, capw = 0, capv = 0;
   std::lock_guard<std::mutex> lock(mu);
   int rc = HSRQ_OK;
   if (!ctx) rc = hsrq_context_create(&ctx, -1);
-  double *dH = nullptr, *dwr = nullptr, *dvr = nullptr;
-  if (!rc) rc = hipMalloc(&dH, (size_t)ldH * n * sizeof(double)) == hipSuccess ? 0 : HSRQ_ENOMEM;
-  if (!rc) rc = hipMalloc(&dwr, (size_t)nrhs * sizeof(double)) == hipSuccess ? 0 : HSRQ_ENOMEM;
-  if (!rc) rc = hipMalloc(&dvr, (size_t)n * nrhs * sizeof(double)) == hipSuccess ? 0 : HSRQ_ENOMEM;
-  if (!rc && hipMemcpy(dH, H, (size_t)ldH * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+  auto grow = [&](double*& p, size_t& cap, size_t need) {
+    if (rc || need <= cap) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, need * sizeof(double)) != hipSuccess) {
+      p = nullptr;
+      rc = HSRQ_ENOMEM;
+      return;
+    }
+    cap = need;
+  };
+  grow(dH, capH, (size_t)n * n);
+  grow(dwr, capw, (size_t)nrhs);
+  grow(dvr, capv, (size_t)n * nrhs);
+  if (!rc && hipMemcpy2D(dH, (size_t)n * sizeof(double), H, (size_t)ldH * sizeof(double),
+                         (size_t)n * sizeof(double), n, hipMemcpyHostToDevice) != hipSuccess)
     rc = HSRQ_EHIP;
   if (!rc && hipMemcpy(dwr, wr, (size_t)nrhs * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     rc = HSRQ_EHIP;
   std::vector<int> status(nrhs, 0);
-  if (!rc) rc = hsrq_solve_device(ctx, n, dH, ldH, nrhs, dwr, dvr, n, tlsz, status.data());
+  if (!rc) rc = hsrq_solve_device(ctx, n, dH, n, nrhs, dwr, dvr, n, tlsz, status.data());
   if (!rc && hipMemcpy2D(vr, (size_t)ldvr * sizeof(double), dvr, (size_t)n * sizeof(double),
                          (size_t)n * sizeof(double), nrhs, hipMemcpyDeviceToHost) != hipSuccess)
     rc = HSRQ_EHIP;
   if (!rc && hipMemcpy(wr, dwr, (size_t)nrhs * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     rc = HSRQ_EHIP;
-  if (dH) (void)hipFree(dH);
-  if (dwr) (void)hipFree(dwr);
-  if (dvr) (void)hipFree(dvr);
   if (rc) {
     std::fprintf(stderr, "tiled_inverse_iteration: %s\n", hsrq_strerror(rc));
     return;

@@ -66,11 +66,52 @@ size_t prob_workspace_bytes(int n, int m, int b);
 // workspace at `ws` (prob_workspace_bytes(q.n, q.m, b) bytes).
 void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws);
 
-// Item tables.  `i` is the problem's index in the descriptor array.
+// Item tables.  `i` is the problem's index in the descriptor array.  The
+// builders append to any container with push_back (std::vector, or the
+// fixed-capacity host buffers of the allocation-free entry point).
 bool norm_block_needed_host(const ProbDev& P, int rb, int cb);
-void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
-                    std::vector<RowItem>& fin);
-void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init);
+template <class GV, class RV>
+void add_norm_items(const ProbDev& P, int i, GV& part, RV& fin) {
+  if (P.m == 0) return;
+  for (int t = 0; t < P.T; ++t) fin.push_back(RowItem{i, t, 0, 0});
+  for (int rb = 0; rb < P.nrb; ++rb)
+    for (int cb = 0; cb < P.nrb; ++cb)
+      if (norm_block_needed_host(P, rb, cb)) part.push_back(GemmItem{i, 0, rb, cb, 1, 0});
+}
+template <class RV>
+void add_init_items(const ProbDev& P, int i, RV& init) {
+  for (int rb = 0; rb < P.nrb; ++rb)
+    for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init.push_back(RowItem{i, rb, k0, 0});
+}
+// Items of tile column J of problem i (J = T-1 ... 0): the diagonal sweep
+// (k1) and the panel pass of the rows above tile J-1 (gb; tile J-1's own
+// rows are updated at the start of its sweep).  The panel pass goes out in
+// row-block pairs, XCD-aware: blocks are dealt to the 8 XCDs round robin, so
+// pairs 8t .. 8t+7 go out interleaved and every XCD sweeps all column blocks
+// of one pair from its own L2.
+template <class KV, class GV>
+void add_step_items(const ProbDev& P, int i, int J, int pack, KV& k1, GV& gb) {
+  if (P.m == 0 || J < 0 || J >= P.T) return;
+  const int k1step = pack == 4 ? 16 : kShiftsPerK1;
+  for (int k0 = 0; k0 < P.m; k0 += k1step) k1.push_back(K1Item{i, J, k0, 0});
+  if (J == 0) return;
+  const int rbn = tile_first_rowblock(P, J - 1);
+  const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
+  const int npairs = (rbn + kGemmRowBlocks - 1) / kGemmRowBlocks;
+  for (int t = 0; t < npairs; t += 8) {
+    const int te = npairs < t + 8 ? npairs : t + 8;
+    for (int cb = 0; cb < ncb; ++cb)
+      for (int q = t; q < te; ++q) {
+        const int rb = q * kGemmRowBlocks;
+        const int nrbk = rbn - rb < kGemmRowBlocks ? rbn - rb : kGemmRowBlocks;
+        gb.push_back(GemmItem{i, J, rb, cb, nrbk, 0});
+      }
+  }
+}
+template <class SV>
+void add_bt_items(const ProbDev& P, int i, SV& bt) {
+  for (int k = 0; k < P.m; ++k) bt.push_back(ShiftItem{i, k, 0, 0});
+}
 struct StepItems {
   std::vector<K1Item> k1;          // diagonal sweep
   std::vector<GemmItem> gb;        // panel pass of the rows above the next diagonal
@@ -80,9 +121,9 @@ struct StepItems {
     gb.clear();
   }
 };
-// Items of tile column J of problem i (J = T-1 ... 0).
-void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out);
-void add_bt_items(const ProbDev& P, int i, std::vector<ShiftItem>& bt);
+inline void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out) {
+  add_step_items(P, i, J, pack, out.k1, out.gb);
+}
 
 // Launchers (hsrq_kernels.hip).  Zero counts launch nothing.
 hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, int, hipStream_t);

@@ -133,44 +133,4 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.status = (int*)(ws + L.st);
 }
 
-void add_norm_items(const ProbDev& P, int i, std::vector<GemmItem>& part,
-                    std::vector<RowItem>& fin) {
-  if (P.m == 0) return;
-  for (int t = 0; t < P.T; ++t) fin.push_back({i, t, 0, 0});
-  for (int rb = 0; rb < P.nrb; ++rb)
-    for (int cb = 0; cb < P.nrb; ++cb)
-      if (norm_block_needed_host(P, rb, cb)) part.push_back({i, 0, rb, cb, 1, 0});
-}
-
-void add_init_items(const ProbDev& P, int i, std::vector<RowItem>& init) {
-  for (int rb = 0; rb < P.nrb; ++rb)
-    for (int k0 = 0; k0 < P.m; k0 += kShiftsPerK1) init.push_back({i, rb, k0, 0});
-}
-
-void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& out) {
-  if (P.m == 0 || J < 0 || J >= P.T) return;
-  const int k1step = pack == 4 ? 16 : kShiftsPerK1;
-  for (int k0 = 0; k0 < P.m; k0 += k1step) out.k1.push_back({i, J, k0, 0});
-  if (J == 0) return;
-  // Panel pass of the rows above tile J-1 (tile J-1's own rows are updated
-  // at the start of its sweep), in row-block pairs, XCD-aware order: blocks
-  // are dealt to the 8 XCDs round robin, so pairs 8t .. 8t+7 go out
-  // interleaved and every XCD sweeps all column blocks of one pair from its
-  // own L2.
-  const int rbn = tile_first_rowblock(P, J - 1);
-  const int ncb = (2 * P.m + kGemmCols - 1) / kGemmCols;
-  std::vector<std::pair<int, int>> pairs;
-  for (int rb = 0; rb < rbn; rb += kGemmRowBlocks)
-    pairs.push_back({rb, std::min(kGemmRowBlocks, rbn - rb)});
-  for (size_t t = 0; t < pairs.size(); t += 8) {
-    const size_t te = std::min(pairs.size(), t + 8);
-    for (int cb = 0; cb < ncb; ++cb)
-      for (size_t q = t; q < te; ++q) out.gb.push_back({i, J, pairs[q].first, cb, pairs[q].second, 0});
-  }
-}
-
-void add_bt_items(const ProbDev& P, int i, std::vector<ShiftItem>& bt) {
-  for (int k = 0; k < P.m; ++k) bt.push_back({i, k, 0, 0});
-}
-
 }  // namespace hsrq

@@ -242,6 +242,9 @@ void finish(hsrq_server* s, Req* r) {
   for (int k = 0; k < r->in.m; ++k) s->eigvecs += (r->status[k] & HSRQ_CONVERGED) ? 1 : 0;
   s->lat_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
   s->t_last = r->t_done;
+  // a client may already wait for r (requests without shifts finish in
+  // enqueue_tick, outside any tick's retirement)
+  s->cv_done.notify_all();
 }
 
 // After the tick in which r's back-transform ran: merge the status, restart

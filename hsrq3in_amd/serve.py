@@ -84,7 +84,13 @@ class Server:
 
     def submit_device(self, Ht, wrt, vrt) -> int:
         """Device request on torch tensors (Ht[j, i] = H(i, j); vrt (m, ldvr))."""
-        n, m = Ht.shape[0], wrt.numel()
+        import torch
+
+        from . import _check_problem
+        n, m = _check_problem(Ht, wrt, vrt)
+        # the server's streams do not wait for torch's: the inputs must be
+        # complete before the worker reads them at admission
+        torch.cuda.current_stream(Ht.device).synchronize()
         req = Request(n, Ht.data_ptr(), Ht.shape[1], m, wrt.data_ptr(), vrt.data_ptr(),
                       vrt.shape[1], ON_DEVICE)
         rid = ctypes.c_int64()

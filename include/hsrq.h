@@ -12,6 +12,14 @@
  *
  * Error convention: functions return 0 (HSRQ_OK) or a negative code;
  * hsrq_strerror() describes it.  No function throws or aborts.
+ *
+ * Input ordering: the context's stream (hsrq_context_stream) and the
+ * server's streams are non-blocking; they do not wait for work on other
+ * streams.  Device inputs must be complete when a call is made: a caller
+ * whose inputs come from another stream synchronizes it first, or makes the
+ * context stream wait for it (hipStreamWaitEvent).  Calls documented as
+ * synchronous return with their outputs written; hsrq_plan_execute* is
+ * asynchronous on the context stream (hsrq_plan_sync).
  */
 #ifndef HSRQ_H_
 #define HSRQ_H_

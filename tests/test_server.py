@@ -174,3 +174,28 @@ def test_server_rejects_oversized(ctx):
     with pytest.raises(HsrqError):
         srv.submit(H, wr)
     srv.close()
+
+
+@pytest.mark.gpu
+def test_request_without_shifts_alone(ctx):
+    """A request with m = 0 completes even when it is the only traffic (its
+    completion happens outside any tick's retirement and must still wake the
+    waiting client)."""
+    from hsrq3in_amd.serve import Server
+    srv = Server(ctx, hbm_bytes=64 << 20, max_active=4)
+    try:
+        H, _ = case(100, seed=77, m=1)
+        rid = srv.submit(H, np.zeros(0))
+        done = {}
+
+        def waiter():
+            done["r"] = srv.wait(rid)
+
+        t = threading.Thread(target=waiter, daemon=True)
+        t.start()
+        t.join(timeout=30.0)
+        assert not t.is_alive(), "wait() for an m = 0 request did not return"
+        vr, wr, st, _ = done["r"]
+        assert vr.shape == (100, 0) and wr.size == 0 and st.size == 0
+    finally:
+        srv.close()

@@ -61,7 +61,8 @@ EXPORTS = [
     "hsrq_context_set_exact_norms", "hsrq_context_set_precision", "hsrq_context_set_lookahead",
     "hsrq_context_set_packing", "hsrq_context_set_lanes",
     "hsrq_strerror",
-    "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_batch_device", "hsrq_plan_create",
+    "hsrq_selftest", "hsrq_solve_device", "hsrq_solve_device_noalloc", "hsrq_solve_batch_device",
+    "hsrq_plan_create",
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
     "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes", "hsrq_autotune",
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
@@ -101,6 +102,9 @@ def lib():
         L.hsrq_selftest.argtypes = [_vp]
         L.hsrq_solve_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp,
                                         _vp, ctypes.c_int, ctypes.c_int, _ip]
+        L.hsrq_solve_device_noalloc.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int,
+                                                ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                                ctypes.c_int, _ip]
         L.hsrq_solve_batch_device.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem),
                                               ctypes.c_int, _ip]
         L.hsrq_plan_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem), ctypes.c_int,

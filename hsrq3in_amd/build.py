@@ -23,7 +23,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_cplx.hip", "hsrq_sched.cpp",
-           "hsrq_api.cpp", "hsrq_server.cpp", "hsrq_blas.cpp"]
+           "hsrq_api.cpp", "hsrq_server.cpp", "hsrq_blas.cpp", "hsrq_noalloc.hip"]
 # rocBLAS runs the back-transformation GEMM (hsrq_blas.cpp)
 LINK = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 

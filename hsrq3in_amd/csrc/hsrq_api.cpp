@@ -799,6 +799,7 @@ int hsrq_context_create(hsrq_context** ctx, int device) {
     delete c;
     return HSRQ_EHIP;
   }
+  noalloc_setup(dev);
   *ctx = c;
   return HSRQ_OK;
 }
@@ -862,6 +863,9 @@ const char* hsrq_strerror(int code) {
     case HSRQ_EBUSY: return "capacity exceeded";
     case HSRQ_ENOTFOUND: return "unknown request";
     case HSRQ_ERANGE: return "result outside the floating-point range";
+    case HSRQ_ESCRATCH:
+      return "problem exceeds the allocation-free scratch (HSRQ_NOALLOC_SCRATCH_BYTES, "
+             "HSRQ_NOALLOC_MAX_SHIFTS)";
     default: return "unknown error";
   }
 }

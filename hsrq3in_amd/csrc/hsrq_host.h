@@ -30,6 +30,9 @@ namespace hsrq {
 hipError_t& last_hip_error();
 // Frees the context's rocBLAS handle (hsrq_blas.cpp).
 void release_blas(hsrq_context* ctx);
+// Device address of the allocation-free entry point's scratch array on the
+// current device (hsrq_noalloc.hip); once per device.
+void noalloc_setup(int device);
 
 #define HIP_TRY(expr)                                                 \
   do {                                                                \

@@ -40,7 +40,9 @@ enum {
     HSRQ_ESELFTEST = -5,   /* device self-test failed                     */
     HSRQ_EBUSY = -6,       /* server queue full / capacity exceeded       */
     HSRQ_ENOTFOUND = -7,   /* unknown request id                          */
-    HSRQ_ERANGE = -8       /* a result would leave the FP64 range         */
+    HSRQ_ERANGE = -8,      /* a result would leave the FP64 range         */
+    HSRQ_ESCRATCH = -9     /* problem too large for the allocation-free
+                              entry point's scratch (see below)            */
 };
 
 /* Per-shift status bits (hsrq_*_status arrays). */
@@ -103,6 +105,27 @@ int hsrq_selftest(hsrq_context *ctx);
  * HSRQ_* bit sets in the final order.  Synchronous. */
 int hsrq_solve_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
                       double *dwr, double *dvr, int ldvr, int tlsz, int *status);
+
+/* ---- allocation-free solve ----
+ * No counterpart in the reference (its tiled_inverse_iteration allocates its
+ * workspace per call, inverse_iteration.c:135-146).  For callers that cannot
+ * allocate device memory: tight serving loops and harnesses that forbid
+ * hipMalloc.  The same solve as hsrq_solve_device (same tile, same kernels,
+ * same restarts; bit-identical results and status), with the workspace, the
+ * item tables and the descriptor in a device array of
+ * HSRQ_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on
+ * until the process ends (set up by hsrq_context_create).  The call makes no
+ * allocation; it runs the serial schedule on the context stream.  Sizes:
+ * precision HSRQ_PREC_FP64 only (else HSRQ_EUNSUPPORTED), at most
+ * HSRQ_NOALLOC_MAX_SHIFTS shifts, and about 32 n m + 8 n^2 / 64 bytes of
+ * workspace (n = 4000, m = 400 fits; n = 8000, m = 800 does not):
+ * HSRQ_ESCRATCH otherwise.  Calls share the GPU's array and are serialized by
+ * a process-wide lock (overlapping calls on one GPU would race on it).
+ * Synchronous. */
+#define HSRQ_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define HSRQ_NOALLOC_MAX_SHIFTS 65536
+int hsrq_solve_device_noalloc(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
+                              double *dwr, double *dvr, int ldvr, int tlsz, int *status);
 
 typedef struct {
     int n;

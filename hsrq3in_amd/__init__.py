@@ -32,6 +32,12 @@ LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
 
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2
+HSRQ_SCALE_SHIFT = 8   # status bits 8-23: binary orders of downscaling of the vector
+
+
+def status_scale(st):
+    """Binary orders of downscaling carried into the back-transform, per shift."""
+    return (np.asarray(st) >> HSRQ_SCALE_SHIFT) & 0xFFFF
 HSRQ_ERANGE = -8    # hsrq_grade_device: a graded entry would leave the FP64 range
 PREC_FP64 = 0
 PREC_FP32 = 1       # FP32 MFMA panel pass, FP64-residual check, FP64 re-solve of failures

@@ -1871,7 +1871,9 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
     // converged iff the unscaled growth ||x|| 2^-emin exceeds 0.1/sqrt(n)
     const bool conv = isfinite(ynrm) && ynrm > ldexp(0.1 / sqrt((double)n), emin);
     ynrm_s = conv ? ynrm : -1.0;
-    if (conv) atomicOr(&P.status[k], 1);
+    // status: bit 0 converged; bits 8-23 the binary orders of downscaling the
+    // vector carried into this back-transform (-emin, exponents are <= 0)
+    atomicOr(&P.status[k], (conv ? 1 : 0) | (min(-emin, 0xFFFF) << 8));
   }
   __syncthreads();
   const double ynrm = ynrm_s;

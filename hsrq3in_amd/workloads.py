@@ -258,6 +258,20 @@ def serving_problem(n: int, seed: int, graded: bool):
 graded_fn = graded
 
 
+def serving_problem_device(ctx, n: int, seed: int, graded: bool, device: str = "cuda"):
+    """``serving_problem`` built in HBM by the device generators (bit-identical
+    to the host build: same parameters, same operations): (Ht, lam) with
+    Ht[j, i] = H(i, j)."""
+    rng = np.random.default_rng(seed)
+    r = np.sqrt(n)
+    lam = rng.uniform(-r, r, size=n)
+    nu, d, u = constructed_parameters(lam, seed + 1, sub=1.0 / n)
+    Ht = ctx.constructed_hessenberg(lam, nu, d, u, device=device)
+    if graded:
+        ctx.grade(Ht, grading_exponents(n, SERVE_KMAX, seed + 2))
+    return Ht, lam
+
+
 def serving_sizes(count: int, seed: int = 5, nmin: int = 500, nmax: int = 16000,
                   graded_frac: float = 0.15):
     """(n, graded, seed) of a pool of serving problems: n log-uniform in

@@ -48,6 +48,13 @@ enum {
 /* Per-shift status bits (hsrq_*_status arrays). */
 #define HSRQ_CONVERGED 1   /* ||x|| growth test passed, x normalised       */
 #define HSRQ_PERTURBED 2   /* a pivot was raised to smin (reference info=1)*/
+/* Bits 8-23 of a real-shift status word: the binary orders of downscaling
+ * the shift's vector carried into the back-transform (minus the smallest
+ * power-of-two tile exponent of its last solve, capped at 65535; 0 when no
+ * protection triggered).  The reference's counterpart is the smallest tile
+ * scaling factor (shifted_Hessenberg_solve.c:123-124). */
+#define HSRQ_SCALE_SHIFT 8
+#define HSRQ_STATUS_SCALE(st) (((st) >> HSRQ_SCALE_SHIFT) & 0xFFFF)
 
 /* Update precision of the panel pass. */
 #define HSRQ_PREC_FP64 0

@@ -43,6 +43,11 @@ def lib():
         L.oracle_tiled_inverse_iteration.argtypes = [
             ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
             _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OracleStats)]
+        L.oracle_tiled_inverse_iteration_ex.restype = ctypes.c_int
+        L.oracle_tiled_inverse_iteration_ex.argtypes = [
+            ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
+            _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OracleStats),
+            ctypes.POINTER(ctypes.c_int), _dp, ctypes.c_int]
         L.oracle_dlartg.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         L.oracle_givens.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.oracle_protect_update.restype = ctypes.c_double
@@ -84,6 +89,30 @@ def tiled_inverse_iteration(H: np.ndarray, wr: np.ndarray, tlsz: int = 64,
     if rc != 0:
         raise ValueError(f"oracle_tiled_inverse_iteration failed: {rc}")
     return vr[:, :m], wr_out, st
+
+
+def tiled_inverse_iteration_detail(H: np.ndarray, wr: np.ndarray, tlsz: int = 64,
+                                   rhs_tlsz: int = 32, nthreads: int = 0,
+                                   max_restarts: int = -1):
+    """As tiled_inverse_iteration, plus per shift (final order): status bits
+    (1 converged, 2 a pivot raised to smin in the last solve) and log2 of the
+    smallest tile scaling factor of the last solve.  max_restarts bounds the
+    restart rounds (-1: n, the reference's limit; 0: first solve only).
+    Returns (vr, wr_out, stats, status, log2scale)."""
+    H = np.asfortranarray(H, dtype=np.float64)
+    n = H.shape[0]
+    wr_out = np.array(wr, dtype=np.float64, copy=True)
+    m = wr_out.size
+    vr = np.zeros((n, max(m, 1)), dtype=np.float64, order="F")
+    status = np.zeros(max(m, 1), dtype=np.int32)
+    lsc = np.zeros(max(m, 1), dtype=np.float64)
+    st = OracleStats()
+    rc = lib().oracle_tiled_inverse_iteration_ex(
+        n, _ptr(H), n, m, _ptr(wr_out), tlsz, rhs_tlsz, _ptr(vr), n, nthreads, ctypes.byref(st),
+        status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(lsc), max_restarts)
+    if rc != 0:
+        raise ValueError(f"oracle_tiled_inverse_iteration_ex failed: {rc}")
+    return vr[:, :m], wr_out, st, status[:m], lsc[:m]
 
 
 def dlartg(f: float, g: float):

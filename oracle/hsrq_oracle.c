@@ -328,7 +328,7 @@ static inline double sc_ratio(sc_t a, sc_t b)
  * last row when J = T-1).
  */
 static void diag_tile(const plan_t *P, int J, double mu, double *v, double *y,
-                      double *c, double *s, sc_t *beta, tstats_t *st)
+                      double *c, double *s, sc_t *beta, tstats_t *st, int *pert)
 {
     const int l = P->fr[J];
     const int b = P->fr[J + 1] - l;
@@ -340,8 +340,10 @@ static void diag_tile(const plan_t *P, int J, double mu, double *v, double *y,
 
         /* y(j) = y(j) / (c(j) v(j) - s(j) H(j,j-1)) */
         double phi = 1.0;
-        st->perturbed += oracle_solve_1x1_real_system(
+        const int info = oracle_solve_1x1_real_system(
             smin, c[j] * v[j], s[j] * Ht(j, j - 1), &y[j], &phi);
+        st->perturbed += info;
+        *pert |= info;
         sc_mul(beta, phi);
         if (phi != 1.0) st->scalings++;
         scale_except(b, j, y, phi);
@@ -384,7 +386,9 @@ static void diag_tile(const plan_t *P, int J, double mu, double *v, double *y,
     }
     /* FIX Q7: the top-left tile has no left neighbour; c[0], s[0] unused. */
     double phi = 1.0;
-    st->perturbed += oracle_solve_1x1_real_system(smin, v[0], 0.0, &y[0], &phi);
+    const int info = oracle_solve_1x1_real_system(smin, v[0], 0.0, &y[0], &phi);
+    st->perturbed += info;
+    *pert |= info;
     sc_mul(beta, phi);
     if (phi != 1.0) st->scalings++;
     scale_vec(b - 1, &y[1], phi);
@@ -573,10 +577,12 @@ HOT static void reduce_offdiag(const plan_t *P, int I, int J, int w, const doubl
  * including the back-transform, norm, convergence test and normalisation
  * of robustTiledSolve (shifted_Hessenberg_solve.c:104-245).
  * vrn: per (shift, tile) norm bounds of the right-hand side, vrn[r + t*w].
- * conv[r] = 1 if converged.
+ * conv[r]: bit 0 converged, bit 1 a pivot was raised to smin.  lsc[r]:
+ * log2 of the smallest tile scale (the factor the vector carries into the
+ * back-transform).
  */
 static void solve_shift_tile(const plan_t *P, int w, const double *wr, double *vr,
-                             int ldvr, double *vrn, int *conv, tstats_t *st)
+                             int ldvr, double *vrn, int *conv, double *lsc, tstats_t *st)
 {
     const int n = P->n, T = P->T;
     const int *fr = P->fr;
@@ -595,6 +601,7 @@ static void solve_shift_tile(const plan_t *P, int w, const double *wr, double *v
 
     for (size_t i = 0; i < (size_t)T * w; i++) scales[i] = sc_one();
     memset(Cc, 0, (size_t)2 * n * w * sizeof(double));
+    int *pert = (int *)calloc((size_t)w, sizeof(int));
 
     /* Initial cross-over columns H(:, n-1) - mu e_{n-1} (reduce.c:239-246, :266-270). */
     for (int r = 0; r < w; r++) {
@@ -608,7 +615,7 @@ static void solve_shift_tile(const plan_t *P, int w, const double *wr, double *v
             memcpy(v, Rt + l + (size_t)r * n, (size_t)b * sizeof(double));
             diag_tile(P, J, wr[r], v, vr + l + (size_t)r * ldvr,
                       Cc + l + (size_t)r * n, Ss + l + (size_t)r * n,
-                      &scales[r + (size_t)J * w], st);
+                      &scales[r + (size_t)J * w], st, &pert[r]);
         }
         for (int I = J - 1; I >= 0; I--) {
             tile_update(P, I, J, w, wr, vr, ldvr, vrn + (size_t)I * w,
@@ -653,7 +660,10 @@ static void solve_shift_tile(const plan_t *P, int w, const double *wr, double *v
         } else {
             conv[r] = 0;
         }
+        conv[r] |= pert[r] ? 2 : 0;
+        lsc[r] = smin.m > 0.0 ? smin.e + log2(smin.m) : -INFINITY;
     }
+    free(pert);
     free(Rt);
     free(scales);
 }
@@ -661,7 +671,7 @@ static void solve_shift_tile(const plan_t *P, int w, const double *wr, double *v
 /* Solve `cnt` contiguous shifts, shift tiles of rhs_tlsz in parallel. */
 static void solve_shifts(const plan_t *P, int cnt, const double *wr, double *vr,
                          int ldvr, const double *tile_bound /* per shift x tile */,
-                         int rhs_tlsz, int *conv, oracle_stats_t *stats)
+                         int rhs_tlsz, int *conv, double *lsc, oracle_stats_t *stats)
 {
     if (cnt <= 0) return;
     const int nt = (cnt + rhs_tlsz - 1) / rhs_tlsz;
@@ -676,7 +686,8 @@ static void solve_shifts(const plan_t *P, int cnt, const double *wr, double *vr,
             for (int r = 0; r < w; r++)
                 vrn[r + (size_t)t * w] = tile_bound[(size_t)(k0 + r) * P->T + t];
         tstats_t st = {0, 0};
-        solve_shift_tile(P, w, wr + k0, vr + (size_t)k0 * ldvr, ldvr, vrn, conv + k0, &st);
+        solve_shift_tile(P, w, wr + k0, vr + (size_t)k0 * ldvr, ldvr, vrn, conv + k0,
+                         lsc + k0, &st);
         sc += st.scalings;
         pe += st.perturbed;
         free(vrn);
@@ -715,6 +726,16 @@ int oracle_tiled_inverse_iteration(int n, const double *H, int ldH, int nrhs,
                                    double *vr, int ldvr, int nthreads,
                                    oracle_stats_t *stats)
 {
+    return oracle_tiled_inverse_iteration_ex(n, H, ldH, nrhs, wr, tlsz, rhs_tlsz, vr, ldvr,
+                                             nthreads, stats, NULL, NULL, -1);
+}
+
+int oracle_tiled_inverse_iteration_ex(int n, const double *H, int ldH, int nrhs,
+                                      double *wr, int tlsz, int rhs_tlsz,
+                                      double *vr, int ldvr, int nthreads,
+                                      oracle_stats_t *stats, int *status, double *log2scale,
+                                      int max_restarts)
+{
     if (n <= 0 || ldH < n || nrhs < 0 || tlsz <= 0 || tlsz % 4 != 0 || rhs_tlsz <= 0
         || ldvr < n || !H || !wr || (nrhs > 0 && !vr))
         return -1;
@@ -751,45 +772,54 @@ int oracle_tiled_inverse_iteration(int n, const double *H, int ldH, int nrhs,
     plan_t P = {n, T, fr, H, ldH, Hn, fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps))};
 
     int *conv = (int *)malloc((size_t)nrhs * sizeof(int));
+    double *lsc = (double *)malloc((size_t)nrhs * sizeof(double));
+    double *tls = (double *)malloc((size_t)nrhs * sizeof(double));
     double *bound = (double *)malloc((size_t)nrhs * T * sizeof(double));
     double *twr = (double *)malloc((size_t)nrhs * sizeof(double));
     double *tvr = (double *)malloc((size_t)nrhs * ldvr * sizeof(double));
     int *tcv = (int *)malloc((size_t)nrhs * sizeof(int));
 
     start_vectors(&P, nrhs, vr, ldvr, eps3, -1, bound);
-    solve_shifts(&P, nrhs, wr, vr, ldvr, bound, rhs_tlsz, conv, stats);
+    solve_shifts(&P, nrhs, wr, vr, ldvr, bound, rhs_tlsz, conv, lsc, stats);
 
     /* FIX Q2/Q3: stable partition [converged | unconverged] of the active
      * segment, then re-solve only the unconverged ones with the its-th
      * restart vector, at most n times (dlaein semantics). */
     int first = 0;
-    for (int its = 0; its < n; its++) {
+    const int rounds = max_restarts < 0 || max_restarts > n ? n : max_restarts;
+    for (int its = 0; its < rounds; its++) {
         int cnt = nrhs - first, nc = 0;
-        for (int r = first; r < nrhs; r++) nc += conv[r];
+        for (int r = first; r < nrhs; r++) nc += conv[r] & 1;
         if (nc == cnt) break;
         int pc = 0, pn = nc;
         for (int r = first; r < nrhs; r++) {
-            const int dst = conv[r] ? pc++ : pn++;
+            const int dst = (conv[r] & 1) ? pc++ : pn++;
             twr[dst] = wr[r];
             tcv[dst] = conv[r];
+            tls[dst] = lsc[r];
             memcpy(tvr + (size_t)dst * ldvr, vr + (size_t)r * ldvr, (size_t)n * sizeof(double));
         }
         for (int q = 0; q < cnt; q++) {
             wr[first + q] = twr[q];
             conv[first + q] = tcv[q];
+            lsc[first + q] = tls[q];
             memcpy(vr + (size_t)(first + q) * ldvr, tvr + (size_t)q * ldvr, (size_t)n * sizeof(double));
         }
         first += nc;
         start_vectors(&P, nrhs - first, vr + (size_t)first * ldvr, ldvr, eps3, its, bound);
         solve_shifts(&P, nrhs - first, wr + first, vr + (size_t)first * ldvr, ldvr, bound,
-                     rhs_tlsz, conv + first, stats);
+                     rhs_tlsz, conv + first, lsc + first, stats);
         if (stats) stats->restarts++;
     }
     if (stats) {
         int nc = 0;
-        for (int r = 0; r < nrhs; r++) nc += !conv[r];
+        for (int r = 0; r < nrhs; r++) nc += !(conv[r] & 1);
         stats->nonconverged = nc;
     }
+    if (status) memcpy(status, conv, (size_t)nrhs * sizeof(int));
+    if (log2scale) memcpy(log2scale, lsc, (size_t)nrhs * sizeof(double));
+    free(lsc);
+    free(tls);
 
     free(conv);
     free(bound);

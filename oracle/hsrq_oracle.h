@@ -33,6 +33,19 @@ int oracle_tiled_inverse_iteration(int n, const double *H, int ldH, int nrhs,
                                    double *vr, int ldvr, int nthreads,
                                    oracle_stats_t *stats);
 
+/* The same, with per-shift outputs in the final shift order (either may be
+ * NULL): status[k] bit 0 converged, bit 1 a pivot of the last solve was
+ * raised to smin (the reference's info = 1, robust.c:462-465);
+ * log2scale[k] = log2 of the shift's smallest tile scaling factor in its last
+ * solve (<= 0; the factor its vector carries into the back-transform,
+ * shifted_Hessenberg_solve.c:123-124).  max_restarts: restart rounds at
+ * most (< 0: n, the reference's limit; 0: the first solve alone). */
+int oracle_tiled_inverse_iteration_ex(int n, const double *H, int ldH, int nrhs,
+                                      double *wr, int tlsz, int rhs_tlsz,
+                                      double *vr, int ldvr, int nthreads,
+                                      oracle_stats_t *stats, int *status,
+                                      double *log2scale, int max_restarts);
+
 /* Building blocks, exported for the oracle's own pinning tests. */
 void oracle_dlartg(double f, double g, double *c, double *s, double *r);
 void oracle_givens(double f, double g, double *c, double *s);

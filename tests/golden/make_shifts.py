@@ -6,7 +6,9 @@ relative 1e-12.  The eigenvalue computation is O(n^3) (minutes at n = 8000),
 so it runs once here and the lists are committed; the matrices themselves are
 rebuilt bit-identically from their seeds wherever they are needed.
 
-    python tests/golden/make_shifts.py [cfg1] [cfg2] [batch]
+    python tests/golden/make_shifts.py [cfg1] [cfg2] [batch] [cfg4]
+
+cfg4 (n = 20000, m = 2000) takes about an hour of host time.
 """
 import os
 import sys
@@ -52,3 +54,5 @@ if __name__ == "__main__":
         single("shifts_n8000_s2.npz", 8000, 800, 2)
     if "batch" in which:
         batch()
+    if "cfg4" in which:
+        single("shifts_n20000_s4.npz", 20000, 2000, 4)

@@ -80,9 +80,14 @@ def test_parity_small(ctx, n):
         assert one_minus_abs_cos(x[:, conv], xo[:, conv]).max() <= TOL
 
 
-@pytest.mark.parametrize("n,kmax,tile", [(200, 60, 64), (200, 400, 64), (500, 400, 128),
-                                         (1000, 400, 64)])
-def test_graded(ctx, n, kmax, tile):
+# Bar 1e-10 (the north star's) for the graded ranges of the workloads.  An
+# intermediate grading (k in [-60, 60]) makes the eigenvalues themselves
+# ill-conditioned in H's norm (left and right eigenvectors D^-T y0, D x0 far
+# from parallel), so two backward-stable runs differ by more there: measured
+# up to 2.3e-10 at n = 1000; that case keeps 1e-9.
+@pytest.mark.parametrize("n,kmax,tile,tol", [(200, 60, 64, 1e-9), (200, 400, 64, 1e-10),
+                                             (500, 400, 128, 1e-10), (1000, 400, 64, 1e-10)])
+def test_graded(ctx, n, kmax, tile, tol):
     H, wr = case(n, seed=n + kmax, m=16, kmax=kmax)
     x, wo, st, res = gpu_solve(ctx, H, wr, tile)
     xo, woo, sto = oracle.tiled_inverse_iteration(H, wr, tile, 8)
@@ -91,7 +96,7 @@ def test_graded(ctx, n, kmax, tile):
     assert res.max() <= 10 * n
     d = one_minus_abs_cos(x, xo)
     print("graded max 1-|cos| vs oracle", d.max())
-    assert d.max() <= 1e-8
+    assert d.max() <= tol
 
 
 def test_cluster(ctx):

@@ -241,3 +241,77 @@ def test_constructed_spectrum_known_eigenvalues():
     assert st.nonconverged == 0
     ns, _ = oracle.residuals(H, wo, x)
     assert ns.max() <= 10 * 400
+
+
+# Shifts of the seed-1 Gaussian Hessenberg of order 30 whose first solve fails
+# the growth test and converges after 1, 2 and 3 restart rounds (found by a
+# scan of relative offsets from the eigenvalues).
+RESTART_CASES = [(3.094879333719913, 1.7782794100389227e-13, 1),
+                 (0.30770389311251, 5.623413251903491e-09, 2),
+                 (-0.423083762617985, 1.7782794100389228e-08, 3)]
+
+
+def dlaein_vector(n: int, eps3: float, its: int) -> np.ndarray:
+    """LAPACK 3.10 dlaein's its-th restart vector (inverse_iteration.c:36-57)."""
+    v = np.full(n, eps3 / (np.sqrt(n) + 1.0))
+    v[0] = eps3
+    v[n - 1 - its] -= eps3 * np.sqrt(n)
+    return v
+
+
+def restart_problem():
+    H = w.gaussian_hessenberg(30, 1)
+    ev = np.linalg.eigvals(H)
+    lam = ev.real[ev.imag == 0]
+    shifts = []
+    for l0, d, _ in RESTART_CASES:
+        l = lam[np.argmin(np.abs(lam - l0))]
+        shifts.append(l * (1 + d))
+    return H, np.array(shifts)
+
+
+def restart_candidates(H, s):
+    """Dense LU solutions of (H - s I) x = b, normalised, for b = eps3 * 1 and
+    dlaein's restart vectors its = 0 .. 3."""
+    n = H.shape[0]
+    eps3 = eps3_of(H)
+    out = []
+    for b in [np.full(n, eps3)] + [dlaein_vector(n, eps3, i) for i in range(4)]:
+        r = np.linalg.solve(H - s * np.eye(n), b)
+        out.append(r / np.linalg.norm(r))
+    return out
+
+
+def check_restart_vector(x, cands, rounds):
+    """x is the solve from the (rounds-1)-th restart vector: 1e-14 from that
+    candidate (entrywise, sign of the dense solve) and >= 5e-13 from every
+    other (measured: 1e-16 against 7e-13 .. 3e-9)."""
+    d = [np.abs(x * np.sign(x @ r) - r).max() for r in cands]
+    assert 1 - abs(x @ cands[rounds]) <= 1e-10
+    assert np.sign(x @ cands[rounds]) > 0
+    assert d[rounds] <= 1e-14, d
+    assert min(d[:rounds] + d[rounds + 1:]) >= 5e-13, d
+
+
+def test_restart_vector_dense_solve():
+    """A shift that converges only after r restart rounds ends with the solve
+    from dlaein's (r-1)-th restart vector: the oracle's vector equals the
+    dense LU solution of (H - mu I) x = v_(r-1), direction and sign, and
+    differs measurably from the solves from the other start vectors."""
+    H, shifts = restart_problem()
+    for (l0, d, rounds), s in zip(RESTART_CASES, shifts):
+        x, wo, st = oracle.tiled_inverse_iteration(H, np.array([s]), 8, 4)
+        assert st.restarts == rounds and st.nonconverged == 0
+        check_restart_vector(x[:, 0], restart_candidates(H, s), rounds)
+
+
+def test_cluster_first_pass_split():
+    """The clustered n = 8000 problem of the plain configuration (200 of 800
+    shifts at relative spacing 1e-8): the first solve converges 510 shifts and
+    leaves 290 to the restart vectors (their left eigenvectors are localised,
+    so eps3 (1, ..., 1) has a small component along them)."""
+    lam, shifts = w.cluster_spectrum(8000, 800, 200, 4)
+    H = w.cluster_hessenberg(lam, 200, 4)
+    x, wo, st, so, lsc = oracle.tiled_inverse_iteration_detail(H, shifts, 128, 16, 0, 0)
+    assert int((so & 1).sum()) == 510 and st.nonconverged == 290
+    assert np.isfinite(x).all()

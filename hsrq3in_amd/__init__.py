@@ -32,6 +32,7 @@ LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
 
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2
+HSRQ_VALIDATED = 4     # server results: residual <= 10 n eps (validated online)
 HSRQ_SCALE_SHIFT = 8   # status bits 8-23: binary orders of downscaling of the vector
 
 
@@ -72,7 +73,7 @@ EXPORTS = [
     "hsrq_plan_execute", "hsrq_plan_execute_graph", "hsrq_plan_execute_profiled",
     "hsrq_plan_sync", "hsrq_plan_status", "hsrq_plan_lanes", "hsrq_autotune",
     "hsrq_plan_workspace_bytes", "hsrq_plan_destroy", "hsrq_context_stream",
-    "hsrq_residuals_device",
+    "hsrq_residuals_device", "hsrq_residuals_batch_device",
     "hsrq_gaussian_hessenberg_device", "hsrq_constructed_hessenberg_device", "hsrq_grade_device",
     "hsrq_backtransform_device", "hsrq_solve_complex_device",
     "hsrq_server_create", "hsrq_server_submit", "hsrq_server_poll", "hsrq_server_wait",
@@ -127,6 +128,7 @@ def lib():
         L.hsrq_plan_destroy.argtypes = [_vp]
         L.hsrq_residuals_device.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int,
                                             _vp, _vp, ctypes.c_int, _dp]
+        L.hsrq_residuals_batch_device.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(Problem), _dp]
         L.hsrq_gaussian_hessenberg_device.argtypes = [_vp, ctypes.c_int, ctypes.c_uint64, _vp,
                                                       ctypes.c_int]
         L.hsrq_constructed_hessenberg_device.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, _dp,
@@ -366,6 +368,23 @@ class Context:
                                            res.ctypes.data_as(_dp)), "hsrq_residuals_device")
         return res[:m]
 
+
+    def residuals_batch(self, items) -> list:
+        """Residuals of many problems in one device pass (hsrq_residuals_batch_device):
+        items as for solve_batch; one array per problem."""
+        arr = self._problems(items)
+        total = sum(w.numel() for _, w, _ in items)
+        res = np.zeros(max(total, 1), dtype=np.float64)
+        if items:
+            _after_torch(self.stream_ptr(), items[0][0].device)
+        _check(lib().hsrq_residuals_batch_device(self._h, len(items), arr,
+                                                 res.ctypes.data_as(_dp)),
+               "hsrq_residuals_batch_device")
+        out, pos = [], 0
+        for _, w, _ in items:
+            out.append(res[pos:pos + w.numel()])
+            pos += w.numel()
+        return out
 
     def solve_complex(self, Ht, wrt, wit, vrt=None, max_restarts: int = 0):
         """Complex shifts wr + i wi (device tensors, m entries) on Ht (as in

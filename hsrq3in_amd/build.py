@@ -22,10 +22,9 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_cplx.hip", "hsrq_sched.cpp",
-           "hsrq_api.cpp", "hsrq_server.cpp", "hsrq_blas.cpp", "hsrq_noalloc.hip"]
-# rocBLAS runs the back-transformation GEMM (hsrq_blas.cpp)
-LINK = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+SOURCES = ["hsrq_kernels.hip", "hsrq_gen.hip", "hsrq_cplx.hip", "hsrq_gemm.hip", "hsrq_sched.cpp",
+           "hsrq_api.cpp", "hsrq_server.cpp", "hsrq_post.cpp", "hsrq_noalloc.hip"]
+LINK = []
 
 
 def _run(cmd):

@@ -612,20 +612,10 @@ int solve_reduced(hsrq_context* ctx, int count, const hsrq_problem* probs, int t
       if (m == 0) continue;
       int* sti = st.data() + off[i];
       // FP64 residuals of the FP32 vectors
-      double *work = nullptr, *dres = nullptr;
-      HIP_TRY(hipMalloc(&work, ((size_t)n * m + 65 * (size_t)n + 4) * sizeof(double)));
-      hipError_t e = hipMalloc(&dres, (size_t)m * sizeof(double));
       std::vector<double> res(m);
-      if (e == hipSuccess) e = launch_residual(n, q.H, q.ldH, m, q.wr, q.vr, q.ldvr, work, dres, stream);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(res.data(), dres, m * sizeof(double), hipMemcpyDeviceToHost, stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(stream);
-      (void)hipFree(work);
-      if (dres) (void)hipFree(dres);
-      if (e != hipSuccess) {
-        last_hip_error() = e;
-        return HSRQ_EHIP;
-      }
+      rc = residuals_run(ctx, 1, &q, res.data());
+      if (rc) return rc;
+      hipError_t e = hipSuccess;
       std::vector<int> bad;
       for (int k = 0; k < m; ++k)
         if (!(sti[k] & HSRQ_CONVERGED) || !(res[k] <= 10.0 * n)) bad.push_back(k);
@@ -807,7 +797,7 @@ int hsrq_context_create(hsrq_context** ctx, int device) {
 void hsrq_context_destroy(hsrq_context* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  release_blas(ctx);
+  release_post(ctx);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1037,31 +1027,6 @@ int hsrq_autotune(hsrq_context* ctx, int count, const hsrq_problem* probs, int* 
   if (packing) *packing = bp;
   if (lanes) *lanes = bl;
   if (ms) *ms = best;
-  return HSRQ_OK;
-}
-
-int hsrq_residuals_device(hsrq_context* ctx, int n, const double* dH, int ldH, int m,
-                          const double* dwr, const double* dvr, int ldvr, double* res) {
-  if (!ctx || n <= 0 || m < 0 || ldH < n || ldvr < n || !dH || (m > 0 && (!dwr || !dvr || !res)))
-    return HSRQ_EINVAL;
-  if (m == 0) return HSRQ_OK;
-  HIP_TRY(hipSetDevice(ctx->device));
-  double* work = nullptr;
-  double* dres = nullptr;
-  // R (n x m), row sums (n), partial row sums (64 n)
-  HIP_TRY(hipMalloc(&work, ((size_t)n * m + 65 * (size_t)n + 4) * sizeof(double)));
-  hipError_t e = hipMalloc(&dres, (size_t)m * sizeof(double));
-  if (e == hipSuccess)
-    e = launch_residual(n, dH, ldH, m, dwr, dvr, ldvr, work, dres, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(res, dres, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(work);
-  if (dres) (void)hipFree(dres);
-  if (e != hipSuccess) {
-    last_hip_error() = e;
-    return HSRQ_EHIP;
-  }
   return HSRQ_OK;
 }
 

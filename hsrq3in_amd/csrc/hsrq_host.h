@@ -22,14 +22,18 @@ struct hsrq_context {
   int cus = 0;           // compute units of the device
   int pack = 0;          // shifts per wavefront in the diagonal sweep: 1, 4 (0: auto)
   int lanes = 0;         // concurrent lanes of a plan (0: auto)
-  void* blas = nullptr;  // rocBLAS handle of the back-transformation, made on first use
+  void* post = nullptr;  // descriptors and items of the products after the solve
+  size_t post_cap = 0;
 };
 
 namespace hsrq {
 
 hipError_t& last_hip_error();
-// Frees the context's rocBLAS handle (hsrq_blas.cpp).
-void release_blas(hsrq_context* ctx);
+// Frees the context's buffer of the products after the solve (hsrq_post.cpp).
+void release_post(hsrq_context* ctx);
+// FP64 residuals of the problems' eigenvectors (host res, sum of m entries;
+// hsrq_post.cpp).  Synchronous.
+int residuals_run(hsrq_context* ctx, int count, const hsrq_problem* probs, double* res);
 // Device address of the allocation-free entry point's scratch array on the
 // current device (hsrq_noalloc.hip); once per device.
 void noalloc_setup(int device);
@@ -139,9 +143,6 @@ hipError_t launch_update32(const ProbDev*, const GemmItem*, int, hipStream_t,
                            int max_blocks = 0);
 hipError_t launch_backtransform(const ProbDev*, const ShiftItem*, int, hipStream_t);
 hipError_t launch_mfma_selftest(int*, hipStream_t);
-hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
-                           const double* X, int64_t ldX, double* work, double* res,
-                           hipStream_t st);
 // ||H(i,:)||_1 per row into rs (n); work: 64 n doubles.
 hipError_t launch_rowsums(int n, const double* H, int64_t ldH, double* work, double* rs,
                           hipStream_t st);

@@ -1883,57 +1883,10 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
 
 
 // ---------------------------------------------------------------------------
-// Validator (test / benchmark only): r_k = (H - wr_k I) x_k for every
-// eigenvector, then ||r_k||_2 / (||H||_oo ||x_k||_2 eps), eps = 2^-52, with
-// overflow-safe (scaled) 2-norms.  H is upper Hessenberg: row i starts at
-// column i-1 (the reference's dense product of validation.c:48-77 adds
-// zeros below the subdiagonal).
+// Absolute row sums of H (its 1-norms per row: ||H||_oo and the row spread
+// of the FP32 mode, the complex path's eps3).  The residual validator is in
+// hsrq_gemm.hip.
 // ---------------------------------------------------------------------------
-// r(:, k) for 64 shifts and 64 rows per workgroup: x chunks of the 64 shifts
-// are staged in LDS, every H element of the row block is read once per 64
-// shifts (columns from the first row's subdiagonal on).
-constexpr int kResShifts = 64;
-__global__ __launch_bounds__(256) void k_resid_rows(int n, const double* __restrict__ H,
-                                                    int64_t ldH, int m,
-                                                    const double* __restrict__ wr,
-                                                    const double* __restrict__ X, int64_t ldX,
-                                                    double* __restrict__ R) {
-  constexpr int RS = kResShifts, XC = 64, PER = RS / 4;
-  __shared__ double xs[RS][XC];
-  const int row0 = blockIdx.x * 64;
-  const int k0 = blockIdx.y * RS;
-  const int rl = threadIdx.x & 63, sg = threadIdx.x >> 6;   // 4 shift groups of 16
-  const int i = row0 + rl;
-  double acc[PER];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) acc[q] = 0.0;
-  const int jstart = row0 > 0 ? row0 - 1 : 0;
-  for (int j0 = jstart; j0 < n; j0 += XC) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < RS * XC; t += 256) {
-      const int s = t / XC, jj = t - s * XC;
-      const int k = k0 + s, j = j0 + jj;
-      xs[s][jj] = (k < m && j < n) ? X[j + (int64_t)k * ldX] : 0.0;
-    }
-    __syncthreads();
-    if (i < n) {
-      const int je = min(XC, n - j0);
-      for (int jj = 0; jj < je; ++jj) {
-        const double h = H[i + (int64_t)(j0 + jj) * ldH];
-#pragma unroll
-        for (int q = 0; q < PER; ++q) acc[q] = fma(h, xs[sg * PER + q][jj], acc[q]);
-      }
-    }
-  }
-  if (i < n) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int k = k0 + sg * PER + q;
-      if (k < m) R[i + (int64_t)k * n] = acc[q] - wr[k] * X[i + (int64_t)k * ldX];
-    }
-  }
-}
-
 // ||H(i,:)||_1 per row in two passes: partial sums over 16 column slabs x 4
 // (64 rows x 4 column groups per workgroup, coalesced down the columns),
 // then a fixed-order sum.  Only the Hessenberg part (j >= i - 1) is read.
@@ -1993,20 +1946,6 @@ __device__ double block_nrm2(int n, const double* __restrict__ v, double* sh) {
   }
   s = block_sum256(s, sh);
   return ldexp(sqrt(s), e);
-}
-
-__global__ __launch_bounds__(256) void k_resid_norms(int n, const double* __restrict__ rs,
-                                                     const double* __restrict__ X, int64_t ldX,
-                                                     const double* __restrict__ R,
-                                                     double* __restrict__ res) {
-  __shared__ double sh[4];
-  const int k = blockIdx.x;
-  double hmax = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) hmax = fmax(hmax, rs[i]);
-  hmax = block_max256(hmax, sh);
-  const double rn = block_nrm2(n, R + (int64_t)k * n, sh);
-  const double xn = block_nrm2(n, X + (int64_t)k * ldX, sh);
-  if (threadIdx.x == 0) res[k] = rn / (hmax * xn * 0x1p-52);
 }
 
 // ---------------------------------------------------------------------------
@@ -2118,24 +2057,6 @@ hipError_t launch_rowsums(int n, const double* H, int64_t ldH, double* work, dou
                      part);
   hipLaunchKernelGGL(k_rowsum_final, dim3((n + 255) / 256), dim3(256), 0, st, n,
                      (const double*)part, rs);
-  return hipGetLastError();
-}
-
-hipError_t launch_residual(int n, const double* H, int64_t ldH, int m, const double* wr,
-                           const double* X, int64_t ldX, double* work, double* res,
-                           hipStream_t st) {
-  if (m <= 0) return hipSuccess;
-  double* R = work;
-  double* rs = work + (int64_t)n * m;
-  double* part = rs + n;   // 4 kRowSlabs x n
-  const int nb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_rowsum_part, dim3((n + 63) / 64, kRowSlabs), dim3(256), 0, st, n, H, ldH,
-                     part);
-  hipLaunchKernelGGL(k_rowsum_final, dim3(nb), dim3(256), 0, st, n, (const double*)part, rs);
-  hipLaunchKernelGGL(k_resid_rows, dim3((n + 63) / 64, (m + kResShifts - 1) / kResShifts),
-                     dim3(256), 0, st, n, H, ldH, m, wr, X, ldX, R);
-  hipLaunchKernelGGL(k_resid_norms, dim3(m), dim3(256), 0, st, n, (const double*)rs, X, ldX,
-                     (const double*)R, res);
   return hipGetLastError();
 }
 

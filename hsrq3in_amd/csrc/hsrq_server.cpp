@@ -15,6 +15,12 @@
 // does not depend on which other requests share a launch: its results are
 // bit-identical alone and in any mix.
 //
+// Every result is validated online: in the tick of a request's
+// back-transform, the residual validator (hsrq_gemm.hip; the reference's
+// validate_real_eigenvectors, validation.c:79-171) computes ||(H - mu I) x||
+// / (||H||_oo ||x|| eps) of its vectors on the device, and the status words
+// returned carry HSRQ_VALIDATED where it is at most 10 n.
+//
 // H, wr, X and the per-(tile, shift) scale exponents stay resident in the
 // arena until the client fetches the result.  Unconverged shifts restart
 // in the same slot with the dlaein vectors (converged first, as the
@@ -23,6 +29,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -33,6 +40,7 @@
 #include <vector>
 
 #include "../../include/hsrq.h"
+#include "hsrq_gemm.h"
 #include "hsrq_host.h"
 
 using namespace hsrq;
@@ -115,6 +123,8 @@ struct Req {
   std::vector<double> wr;      // host copy, final order
   std::vector<int> status;     // host copy, final order
   int* st_pinned = nullptr;    // status of the last solve (pinned)
+  double* res_pinned = nullptr;  // its residuals (pinned)
+  std::vector<double> res;     // host copy, final order
   bool fresh = false;          // needs norms + init in the next tick
   bool waited = false;         // a client waits for (or fetched) the result
   int rc = HSRQ_OK;
@@ -132,12 +142,14 @@ struct hsrq_server {
   void* d_arena = nullptr;
   Arena arena;
   ProbDev* d_probs = nullptr;           // max_active descriptors
+  GemmDesc* d_gdesc = nullptr;          // max_active validator descriptors
   std::vector<int> free_slots;
   // tick buffers, double buffered
   char* h_items[2] = {nullptr, nullptr};
   void* d_items[2] = {nullptr, nullptr};
   size_t cap_items[2] = {0, 0};
   ProbDev* h_probs[2] = {nullptr, nullptr};
+  GemmDesc* h_gdesc[2] = {nullptr, nullptr};
   hipEvent_t ev_tick[2] = {nullptr, nullptr};
   bool tick_pending[2] = {false, false};
   std::vector<Req*> finishing[2];
@@ -151,7 +163,8 @@ struct hsrq_server {
   bool stop = false;
   std::thread worker;
   // statistics
-  int64_t submitted = 0, completed = 0, eigvecs = 0, ticks = 0;
+  int64_t submitted = 0, completed = 0, eigvecs = 0, ticks = 0, validated = 0;
+  double worst_residual = 0.0;   // max residual / (10 n) of completed requests
   std::vector<double> lat_ms;
   Clock::time_point t_first, t_last;
   bool any = false;
@@ -160,8 +173,9 @@ struct hsrq_server {
 
 namespace {
 
+// Workspace of the request's solve, then of its validation.
 size_t footprint(const hsrq_request& r, int b) {
-  size_t f = prob_workspace_bytes(r.n, r.m, b);
+  size_t f = prob_workspace_bytes(r.n, r.m, b) + align_up(resid_workspace_doubles(r.n, r.m) * 8);
   if (r.location == HSRQ_ON_HOST)
     f += align_up((size_t)r.n * r.n * sizeof(double)) + align_up((size_t)r.m * sizeof(double)) +
          align_up((size_t)r.n * r.m * sizeof(double));
@@ -239,7 +253,17 @@ void finish(hsrq_server* s, Req* r) {
   r->st = St::Done;
   r->t_done = Clock::now();
   s->completed += 1;
-  for (int k = 0; k < r->in.m; ++k) s->eigvecs += (r->status[k] & HSRQ_CONVERGED) ? 1 : 0;
+  const double bound = 10.0 * r->in.n;
+  for (int k = 0; k < r->in.m; ++k) {
+    s->eigvecs += (r->status[k] & HSRQ_CONVERGED) ? 1 : 0;
+    if (r->res[k] <= bound) {
+      r->status[k] |= HSRQ_VALIDATED;
+      s->validated += 1;
+    }
+    if (r->status[k] & HSRQ_CONVERGED)
+      s->worst_residual = std::max(s->worst_residual, std::isnan(r->res[k]) ? INFINITY
+                                                                            : r->res[k] / bound);
+  }
   s->lat_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
   s->t_last = r->t_done;
   // a client may already wait for r (requests without shifts finish in
@@ -251,7 +275,10 @@ void finish(hsrq_server* s, Req* r) {
 // the unconverged shifts or finish (worker thread, lock held).
 int complete(hsrq_server* s, Req* r) {
   const int m = r->in.m, cnt = m - r->first;
-  for (int t = 0; t < cnt; ++t) r->status[r->first + t] = r->st_pinned[t];
+  for (int t = 0; t < cnt; ++t) {
+    r->status[r->first + t] = r->st_pinned[t];
+    r->res[r->first + t] = r->res_pinned[t];
+  }
   int nc = 0;
   for (int t = r->first; t < m; ++t) nc += (r->status[t] & HSRQ_CONVERGED) ? 1 : 0;
   const int its_next = r->its + 1;
@@ -266,7 +293,7 @@ int complete(hsrq_server* s, Req* r) {
     if (r->status[t] & HSRQ_CONVERGED) order.push_back(t);
   for (int t = r->first; t < m; ++t)
     if (!(r->status[t] & HSRQ_CONVERGED)) order.push_back(t);
-  std::vector<double> wr_new(r->wr);
+  std::vector<double> wr_new(r->wr), res_new(r->res);
   std::vector<int> st_new(r->status);
   double* tmp = (double*)r->ws;
   const int n = r->in.n;
@@ -274,6 +301,7 @@ int complete(hsrq_server* s, Req* r) {
     const int src = order[t];
     wr_new[r->first + t] = r->wr[src];
     st_new[r->first + t] = r->status[src];
+    res_new[r->first + t] = r->res[src];
     HIP_TRY(hipMemcpyAsync(tmp + (size_t)t * n, r->dvr + (size_t)src * r->ldvr, n * sizeof(double),
                            hipMemcpyDeviceToDevice, s->stream));
   }
@@ -282,6 +310,7 @@ int complete(hsrq_server* s, Req* r) {
                            s->stream));
   r->wr = wr_new;
   r->status = st_new;
+  r->res = res_new;
   HIP_TRY(hipMemcpyAsync(r->dwr, r->wr.data(), m * sizeof(double), hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   r->first += nc;
@@ -296,6 +325,8 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   std::vector<GemmItem> part;
   std::vector<RowItem> fin, init;
   std::vector<ShiftItem> bt;
+  std::vector<GemmTileItem> vrow, vtile, vfin;   // validator of the finishing requests
+  std::vector<GemmDesc> gdescs;
   std::map<int, StepItems> steps;   // by tile size
   std::vector<Req*> fresh, fin_reqs;
   int shifts = 0;
@@ -328,6 +359,16 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
     if (r->J == 0) {
       add_bt_items(P, r->slot, bt);
       fin_reqs.push_back(r);
+      // validation of this solve's vectors (shifts first .. m-1)
+      GemmDesc G;
+      const int mm = r->in.m - r->first;
+      resid_describe(G, r->in.n, r->dH, r->ldH, mm, r->dwr + r->first,
+                     r->dvr + (size_t)r->first * r->ldvr, r->ldvr,
+                     (double*)(r->ws + prob_workspace_bytes(r->in.n, r->in.m, r->b)));
+      gdescs.push_back(G);
+      add_rowmax_items(r->slot, r->in.n, vrow);
+      add_gemm_items(r->slot, r->in.n, mm, kGemmColsRes, vtile);
+      add_resid_final_items(r->slot, mm, vfin);
     }
     if (r->fresh) descs.push_back(P);
   }
@@ -335,7 +376,10 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
 
   // pack the item tables into the pinned staging buffer of this parity
   size_t need = align_up(part.size() * sizeof(GemmItem)) + align_up(fin.size() * sizeof(RowItem)) +
-                align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem));
+                align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem)) +
+                align_up(vrow.size() * sizeof(GemmTileItem)) +
+                align_up(vtile.size() * sizeof(GemmTileItem)) +
+                align_up(vfin.size() * sizeof(GemmTileItem));
   for (auto& kv : steps)
     need += align_up(kv.second.k1.size() * sizeof(K1Item)) +
             align_up(kv.second.gb.size() * sizeof(GemmItem));
@@ -369,6 +413,12 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   const RowItem* d_fin = (const RowItem*)put(fin.data(), fin.size() * sizeof(RowItem));
   const RowItem* d_init = (const RowItem*)put(init.data(), init.size() * sizeof(RowItem));
   const ShiftItem* d_bt = (const ShiftItem*)put(bt.data(), bt.size() * sizeof(ShiftItem));
+  const GemmTileItem* d_vrow =
+      (const GemmTileItem*)put(vrow.data(), vrow.size() * sizeof(GemmTileItem));
+  const GemmTileItem* d_vtile =
+      (const GemmTileItem*)put(vtile.data(), vtile.size() * sizeof(GemmTileItem));
+  const GemmTileItem* d_vfin =
+      (const GemmTileItem*)put(vfin.data(), vfin.size() * sizeof(GemmTileItem));
   struct Dev {
     int b, pack;
     const K1Item* k1;
@@ -400,6 +450,12 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
                        hipMemcpyHostToDevice, s->stream);
     if (e != hipSuccess) return fail(e);
   }
+  for (size_t i = 0; i < gdescs.size(); ++i) {
+    s->h_gdesc[par][i] = gdescs[i];
+    e = hipMemcpyAsync(s->d_gdesc + fin_reqs[i]->slot, s->h_gdesc[par] + i, sizeof(GemmDesc),
+                       hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) return fail(e);
+  }
   // wait for the copies of H and wr of the fresh requests
   for (Req* r : fresh)
     if (r->in.location == HSRQ_ON_HOST && r->its < 0) {
@@ -415,11 +471,18 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   for (const Dev& x : dev)
     if ((e = launch_update(P, x.gb, x.ngb, s->stream)) != hipSuccess) return fail(e);
   if ((e = launch_backtransform(P, d_bt, (int)bt.size(), s->stream)) != hipSuccess) return fail(e);
-  for (Req* r : fin_reqs) {
+  if ((e = launch_resid(s->d_gdesc, d_vrow, (int)vrow.size(), d_vtile, (int)vtile.size(), d_vfin,
+                        (int)vfin.size(), s->stream)) != hipSuccess)
+    return fail(e);
+  for (size_t i = 0; i < fin_reqs.size(); ++i) {
+    Req* r = fin_reqs[i];
     ProbDev Q;
     describe(s, r, Q);
     e = hipMemcpyAsync(r->st_pinned, Q.status, (r->in.m - r->first) * sizeof(int),
                        hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r->res_pinned, gdescs[i].res, (r->in.m - r->first) * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream);
     if (e != hipSuccess) return fail(e);
   }
   if ((e = hipEventRecord(s->ev_tick[par], s->stream)) != hipSuccess) return fail(e);
@@ -511,6 +574,7 @@ void release(hsrq_server* s, Req* r) {
   s->by_id.erase(r->id);
   if (r->ev_copy) (void)hipEventDestroy(r->ev_copy);
   if (r->st_pinned) (void)hipHostFree(r->st_pinned);
+  if (r->res_pinned) (void)hipHostFree(r->res_pinned);
   delete r;
   s->cv.notify_all();   // memory became free: admission may proceed
 }
@@ -539,8 +603,11 @@ int hsrq_server_create(hsrq_context* ctx, size_t hbm_bytes, int max_active, int 
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&s->d_arena, hbm_bytes);
   if (e == hipSuccess) e = hipMalloc((void**)&s->d_probs, (size_t)max_active * sizeof(ProbDev));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->d_gdesc, (size_t)max_active * sizeof(GemmDesc));
   for (int p = 0; p < 2 && e == hipSuccess; ++p) {
     e = hipHostMalloc((void**)&s->h_probs[p], (size_t)max_active * sizeof(ProbDev), 0);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&s->h_gdesc[p], (size_t)max_active * sizeof(GemmDesc), 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_tick[p], hipEventDisableTiming);
   }
   if (e != hipSuccess) {
@@ -575,6 +642,7 @@ int hsrq_server_submit(hsrq_server* s, const hsrq_request* req, int64_t* id) {
   }
   r->wr.resize(q.m);
   r->status.assign(q.m, 0);
+  r->res.assign(q.m, NAN);
   hipError_t e = hipSetDevice(s->device);
   if (e == hipSuccess && q.m > 0) {
     if (q.location == HSRQ_ON_HOST)
@@ -583,10 +651,13 @@ int hsrq_server_submit(hsrq_server* s, const hsrq_request* req, int64_t* id) {
       e = hipMemcpy(r->wr.data(), q.wr, q.m * sizeof(double), hipMemcpyDeviceToHost);
   }
   if (e == hipSuccess) e = hipHostMalloc((void**)&r->st_pinned, std::max(q.m, 1) * sizeof(int), 0);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&r->res_pinned, std::max(q.m, 1) * sizeof(double), 0);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_copy, hipEventDisableTiming);
   if (e != hipSuccess) {
     last_hip_error() = e;
     if (r->st_pinned) (void)hipHostFree(r->st_pinned);
+    if (r->res_pinned) (void)hipHostFree(r->res_pinned);
     if (r->ev_copy) (void)hipEventDestroy(r->ev_copy);
     delete r;
     return HSRQ_EHIP;
@@ -594,6 +665,7 @@ int hsrq_server_submit(hsrq_server* s, const hsrq_request* req, int64_t* id) {
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->worker_rc) {
     (void)hipHostFree(r->st_pinned);
+    (void)hipHostFree(r->res_pinned);
     (void)hipEventDestroy(r->ev_copy);
     delete r;
     return s->worker_rc;
@@ -664,6 +736,8 @@ int hsrq_server_get_stats(hsrq_server* s, hsrq_server_stats* out) {
   out->submitted = s->submitted;
   out->completed = s->completed;
   out->eigvecs = s->eigvecs;
+  out->validated = s->validated;
+  out->worst_residual = s->worst_residual;
   out->ticks = s->ticks;
   out->queued = (int64_t)s->queue.size();
   int64_t active = 0;
@@ -709,15 +783,18 @@ void hsrq_server_destroy(hsrq_server* s) {
   for (Req* r : s->queue) {
     if (r->ev_copy) (void)hipEventDestroy(r->ev_copy);
     if (r->st_pinned) (void)hipHostFree(r->st_pinned);
+    if (r->res_pinned) (void)hipHostFree(r->res_pinned);
     delete r;
   }
   for (int p = 0; p < 2; ++p) {
     if (s->h_items[p]) (void)hipHostFree(s->h_items[p]);
     if (s->d_items[p]) (void)hipFree(s->d_items[p]);
     if (s->h_probs[p]) (void)hipHostFree(s->h_probs[p]);
+    if (s->h_gdesc[p]) (void)hipHostFree(s->h_gdesc[p]);
     if (s->ev_tick[p]) (void)hipEventDestroy(s->ev_tick[p]);
   }
   if (s->d_probs) (void)hipFree(s->d_probs);
+  if (s->d_gdesc) (void)hipFree(s->d_gdesc);
   if (s->d_arena) (void)hipFree(s->d_arena);
   if (s->copy) (void)hipStreamDestroy(s->copy);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -38,7 +38,8 @@ class Stats(ctypes.Structure):
                 ("queued", ctypes.c_int64), ("ticks", ctypes.c_int64),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("span_s", ctypes.c_double), ("hbm_bytes", ctypes.c_size_t),
-                ("hbm_in_use", ctypes.c_size_t), ("hbm_peak", ctypes.c_size_t)]
+                ("hbm_in_use", ctypes.c_size_t), ("hbm_peak", ctypes.c_size_t),
+                ("validated", ctypes.c_int64), ("worst_residual", ctypes.c_double)]
 
 
 def _bind():

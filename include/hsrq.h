@@ -48,6 +48,8 @@ enum {
 /* Per-shift status bits (hsrq_*_status arrays). */
 #define HSRQ_CONVERGED 1   /* ||x|| growth test passed, x normalised       */
 #define HSRQ_PERTURBED 2   /* a pivot was raised to smin (reference info=1)*/
+#define HSRQ_VALIDATED 4   /* request server: residual <= 10 n eps (checked
+                              online by the device validator)               */
 /* Bits 8-23 of a real-shift status word: the binary orders of downscaling
  * the shift's vector carried into the back-transform (minus the smallest
  * power-of-two tile exponent of its last solve, capped at 65535; 0 when no
@@ -65,7 +67,7 @@ enum {
 typedef struct hsrq_context hsrq_context;
 
 /* Context = device + stream + workspace cache (and, made on first use, the
- * rocBLAS handle of the back-transformation).  device < 0: current device.
+ * descriptor buffer of the back-transformation and the validator).  device < 0: current device.
  * One host thread at a time per context; use one context per thread.  The
  * request server (below) is the thread-safe entry for concurrent callers. */
 int hsrq_context_create(hsrq_context **ctx, int device);
@@ -200,7 +202,9 @@ void *hsrq_context_stream(hsrq_context *ctx);
  * bytes, plus H, wr and X when they arrive in host memory.  H, X and the
  * scale exponents stay in HBM until the result is fetched.  A request's
  * results are bit-identical to those of hsrq_solve_device on the same input
- * (same tile), alone or in any mix.
+ * (same tile), alone or in any mix.  Every result is validated on the device
+ * in the tick that completes it (hsrq_residuals_device's validator): status
+ * bit HSRQ_VALIDATED marks residuals <= 10 n eps.
  * ------------------------------------------------------------------------ */
 #define HSRQ_ON_HOST 0     /* H, wr, vr are host pointers (copied in/out)  */
 #define HSRQ_ON_DEVICE 1   /* H, wr, vr are device pointers, used in place */
@@ -221,6 +225,8 @@ typedef struct {
     double p50_ms, p99_ms;   /* submit-to-completion latency percentiles   */
     double span_s;           /* first submission to last completion        */
     size_t hbm_bytes, hbm_in_use, hbm_peak;
+    int64_t validated;       /* eigenvectors with residual <= 10 n eps     */
+    double worst_residual;   /* max over converged vectors of residual/(10 n) */
 } hsrq_server_stats;
 
 typedef struct hsrq_server hsrq_server;
@@ -245,9 +251,17 @@ int hsrq_server_get_stats(hsrq_server *srv, hsrq_server_stats *stats);
 void hsrq_server_destroy(hsrq_server *srv);
 
 /* Residual ||(H - wr_k I) x_k||_2 / (||H||_oo ||x_k||_2 eps), eps = 2^-52,
- * per eigenvector, computed on the device (validator).  res: host, m. */
+ * per eigenvector, computed on the device (validator; the reference's
+ * validate_real_eigenvectors, validation.c:79-171, at scale): the Hessenberg
+ * product H X on FP64 MFMA, reduced per 128-row panel to overflow-safe sums
+ * of squares (LAPACK 3.10 dnrm2's three accumulators).  res: host, m.
+ * Synchronous. */
 int hsrq_residuals_device(hsrq_context *ctx, int n, const double *dH, int ldH, int m,
                           const double *dwr, const double *dvr, int ldvr, double *res);
+/* The same for any number of problems in three launches; res (host) receives
+ * sum(m) residuals in problem order. */
+int hsrq_residuals_batch_device(hsrq_context *ctx, int count, const hsrq_problem *probs,
+                                double *res);
 
 /* ---- complex shifts: eigenvectors of complex-conjugate eigenpairs ----
  * sigma_k = wr[k] + i wi[k] (device arrays, m entries).  One inverse-iteration
@@ -300,8 +314,9 @@ int hsrq_grade_device(hsrq_context *ctx, int n, double *dH, int ldH, const int *
  * a Hessenberg reduction) the columns of V are the eigenvectors of A for the
  * eigenvectors X of H (unit 2-norm stays unit up to rounding).  Device
  * column-major Z (n x n), X and V (n x m); V must not overlap Z or X.  One
- * FP64 GEMM (rocBLAS) enqueued on the context stream.  The reference stops
- * at the eigenvectors of H (inverse_iteration.h:19-22). */
+ * FP64 MFMA GEMM of this library enqueued on the context stream
+ * (asynchronous).  The reference stops at the eigenvectors of H
+ * (inverse_iteration.h:19-22). */
 int hsrq_backtransform_device(hsrq_context *ctx, int n, int m, const double *dZ, int ldZ,
                               const double *dX, int ldX, double *dV, int ldV);
 

@@ -11,7 +11,7 @@ import time
 import numpy as np
 import pytest
 
-from hsrq3in_amd import HSRQ_CONVERGED, HsrqError
+from hsrq3in_amd import HSRQ_CONVERGED, HSRQ_VALIDATED, HsrqError
 from tests.helpers import case
 
 torch = pytest.importorskip("torch")
@@ -66,13 +66,17 @@ def test_server_equals_plain_solve(ctx, problems):
         vr, wo, st, ms = srv.wait(rid)
         xp, wop, stp = _plain(ctx, H, wr)
         assert np.array_equal(wo, wop)
-        assert np.array_equal(st, stp)
+        assert np.array_equal(st & ~HSRQ_VALIDATED, stp)
         assert np.array_equal(vr, xp)
         assert (st & HSRQ_CONVERGED).all()
+        # validated online: the device residual of every vector <= 10 n
+        assert (st & HSRQ_VALIDATED).all()
         assert ms > 0
     s = srv.stats()
     assert s["completed"] == len(problems) and s["active"] == 0 and s["queued"] == 0
     assert s["p99_ms"] >= s["p50_ms"] > 0
+    assert s["validated"] == sum(len(wr) for _, wr in problems)
+    assert 0.0 < s["worst_residual"] <= 1.0
     srv.close()
 
 
@@ -142,8 +146,11 @@ def test_server_restarts_like_plain(ctx):
     vr, wo, st, _ = srv.wait(srv.submit(H, mixed))
     srv.close()
     xp, wop, stp = _plain(ctx, H, mixed)
-    assert np.array_equal(wo, wop) and np.array_equal(st, stp)
+    assert np.array_equal(wo, wop) and np.array_equal(st & ~HSRQ_VALIDATED, stp)
     assert np.array_equal(vr, xp)
+    # the two shifts between eigenvalues never converge and fail validation
+    conv = (st & HSRQ_CONVERGED) != 0
+    assert np.array_equal((st & HSRQ_VALIDATED) != 0, conv)
 
 
 @pytest.mark.gpu

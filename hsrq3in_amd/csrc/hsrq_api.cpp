@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -54,6 +55,21 @@ struct Lane {
   int side_blocks = 0;            // grid cap of the panel passes on `side`
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
   std::vector<hipEvent_t> ev_p, ev_b;
+  bool in_arena = false;          // d_probs, d_items carved from a caller's buffer
+};
+
+// Bump allocator over a buffer the caller keeps (plans built repeatedly on
+// the same memory: the restart rounds).
+struct DevArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t b = align_up(std::max<size_t>(bytes, 1));
+    if (used + b > cap) return nullptr;
+    void* p = base + used;
+    used += b;
+    return p;
+  }
 };
 
 void lane_free(Lane* L) {
@@ -65,8 +81,10 @@ void lane_free(Lane* L) {
   if (L->ev_done) (void)hipEventDestroy(L->ev_done);
   if (L->side) (void)hipStreamDestroy(L->side);
   if (L->own_stream && L->stream) (void)hipStreamDestroy(L->stream);
-  if (L->d_probs) (void)hipFree(L->d_probs);
-  if (L->d_items) (void)hipFree(L->d_items);
+  if (!L->in_arena) {
+    if (L->d_probs) (void)hipFree(L->d_probs);
+    if (L->d_items) (void)hipFree(L->d_items);
+  }
   delete L;
 }
 
@@ -86,6 +104,7 @@ struct hsrq_plan {
   int total_m = 0;
   void* d_work = nullptr;
   size_t work_bytes = 0;
+  bool own_work = true;                 // false: d_work is in a caller's arena
   hipEvent_t ev_fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -99,7 +118,7 @@ void plan_free(hsrq_plan* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   for (Lane* L : p->lanes) lane_free(L);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
-  if (p->d_work) (void)hipFree(p->d_work);
+  if (p->own_work && p->d_work) (void)hipFree(p->d_work);
   delete p;
 }
 
@@ -232,7 +251,7 @@ std::vector<int> step_offsets(const std::vector<ProbDev>& probs, int steps) {
   return off;
 }
 
-int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
+int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single, DevArena* ar) {
   L->pack = choose_pack(ctx->pack, L->b, total_lane_shifts);
   std::vector<GemmItem> part;
   std::vector<RowItem> fin, init;
@@ -265,8 +284,15 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
               align_up(x.gb.size() * sizeof(GemmItem));
   nbytes = std::max(nbytes, kAlign);
   std::vector<char> host(nbytes);
-  HIP_TRY(hipMalloc(&L->d_items, nbytes));
-  HIP_TRY(hipMalloc(&L->d_probs, std::max<size_t>(cnt, 1) * sizeof(ProbDev)));
+  if (ar) {
+    L->in_arena = true;
+    L->d_items = ar->take(nbytes);
+    L->d_probs = (ProbDev*)ar->take(std::max<size_t>(cnt, 1) * sizeof(ProbDev));
+    if (!L->d_items || !L->d_probs) return HSRQ_ENOMEM;
+  } else {
+    HIP_TRY(hipMalloc(&L->d_items, nbytes));
+    HIP_TRY(hipMalloc(&L->d_probs, std::max<size_t>(cnt, 1) * sizeof(ProbDev)));
+  }
   size_t pos = 0;
   auto put = [&](const void* src, size_t sz) -> const void* {
     if (sz) std::memcpy(host.data() + pos, src, sz);
@@ -312,9 +338,11 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single) {
 }
 
 // Builds descriptors, one workspace allocation and the lanes.  its: start
-// vector (-1: eps3 * 1; >= 0: the its-th restart vector; -2: given in vr).
+// vector (-1: eps3 * 1; >= 0: the its-th restart vector; -2: given in vr),
+// or per shift in its_k_of[problem] (device arrays; null entries: its).  ar:
+// build workspace and item tables in this arena instead of allocating them.
 int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, int its,
-               hsrq_plan** out) {
+               hsrq_plan** out, const int* const* its_k_of = nullptr, DevArena* ar = nullptr) {
   *out = nullptr;
   if (!ctx || count < 0 || (count > 0 && !in)) return HSRQ_EINVAL;
   for (int i = 0; i < count; ++i) {
@@ -345,7 +373,14 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       bytes += prob_workspace_bytes(in[pc.prob].n, pc.m, tiles[l]);
     }
   p->work_bytes = bytes;
-  hipError_t e = hipMalloc(&p->d_work, std::max(bytes, kAlign));
+  hipError_t e = hipSuccess;
+  if (ar) {
+    p->own_work = false;
+    p->d_work = ar->take(std::max(bytes, kAlign));
+    if (!p->d_work) e = hipErrorOutOfMemory;
+  } else {
+    e = hipMalloc(&p->d_work, std::max(bytes, kAlign));
+  }
   if (e != hipSuccess) {
     last_hip_error() = e;
     plan_free(p);
@@ -371,11 +406,12 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       q.wr = q.wr + pc.k0;
       q.vr = q.vr + (size_t)pc.k0 * q.ldvr;
       ProbDev P;
-      prob_describe(P, q, L->b, its, ctx->exact, (char*)p->d_work + ws_off[l][j]);
+      const int* itk = its_k_of && its_k_of[pc.prob] ? its_k_of[pc.prob] + pc.k0 : nullptr;
+      prob_describe(P, q, L->b, its, ctx->exact, (char*)p->d_work + ws_off[l][j], itk);
       L->probs.push_back(P);
       shifts += pc.m;
     }
-    rc = lane_build(ctx, L, shifts, groups.size() == 1);
+    rc = lane_build(ctx, L, shifts, groups.size() == 1, ar);
   }
   if (!rc) {
     e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
@@ -513,10 +549,247 @@ int plan_read_status(hsrq_plan* p, int* status) {
   return HSRQ_OK;
 }
 
-// Solves a set of problems with restarts: unconverged shifts of each problem
-// are moved behind the converged ones (stable, wr and vr together) and
-// solved again with the its-th restart vector, at most n times (dlaein
-// semantics; the reference's intent at inverse_iteration.c:160-328).
+// Restarts of the shifts a first solve left unconverged (dlaein semantics,
+// the reference's intent at inverse_iteration.c:160-328, quirks Q2/Q3):
+// round its = 0, 1, ... solves the still unconverged shifts again from the
+// its-th restart vector, at most n rounds; a shift keeps the vector of the
+// round it converged in (or of round n-1), and each round moves the shifts
+// that converged in it behind the ones before, stable.
+//
+// The rounds of one shift are independent of each other given H and the
+// shift, so they are evaluated speculatively: a round of this engine solves,
+// for every pending shift of every problem, the next K start vectors at once
+// (the shift repeated K times with per-shift start vectors, one plan over all
+// problems), and each shift takes the first of them that converges.  K is
+// the budget of solves per round over the pending shifts, so few pending
+// shifts advance many start vectors per round.  Results, order and status
+// are those of the one-round-at-a-time loop (each shift's arithmetic does not
+// depend on what shares its launch).  Memory: one scratch allocation for the
+// whole restart phase and one arena for the plans, grown when a round needs
+// more; no allocation per round otherwise.
+int restart_rounds(hsrq_context* ctx, int count, const hsrq_problem* probs,
+                   const std::vector<int>& tile_of, const std::vector<int>& off,
+                   std::vector<int>& st) {
+  struct Tail {
+    int prob = 0, first = 0, cnt = 0, its0 = 0;
+    std::vector<double> wr;          // the tail's shifts (tail order)
+    std::vector<int> pend;           // tail indices still unresolved
+    std::vector<int> its_pass;       // round of convergence (INT_MAX: never)
+    std::vector<int> stat;           // status of the kept solve
+    double* dbuf = nullptr;          // kept vectors, n x cnt (scratch)
+    double* dwr = nullptr;           // this round's shifts (scratch)
+    int* dits = nullptr;             // this round's start vectors (scratch)
+    double* dy = nullptr;            // this round's vectors, n x mr (scratch)
+    int K = 0, mr = 0;
+  };
+  std::vector<Tail> tails;
+  hipStream_t stream = ctx->stream;
+  // stable partition converged-first of each problem (converged columns move
+  // forward in place; the tail columns are rewritten at the end)
+  for (int i = 0; i < count; ++i) {
+    const hsrq_problem& q = probs[i];
+    int* sti = st.data() + off[i];
+    int nc = 0;
+    for (int k = 0; k < q.m; ++k) nc += (sti[k] & HSRQ_CONVERGED) ? 1 : 0;
+    if (nc == q.m) continue;
+    std::vector<double> wr_h(q.m);
+    HIP_TRY(hipMemcpy(wr_h.data(), q.wr, q.m * sizeof(double), hipMemcpyDeviceToHost));
+    Tail t;
+    t.prob = i;
+    t.first = nc;
+    t.cnt = q.m - nc;
+    std::vector<int> st_new(q.m);
+    std::vector<double> wr_new(q.m);
+    int pc = 0;
+    for (int k = 0; k < q.m; ++k)
+      if (sti[k] & HSRQ_CONVERGED) {
+        if (pc != k)
+          HIP_TRY(hipMemcpyAsync(q.vr + (size_t)pc * q.ldvr, q.vr + (size_t)k * q.ldvr,
+                                 q.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        st_new[pc] = sti[k];
+        wr_new[pc++] = wr_h[k];
+      }
+    for (int k = 0; k < q.m; ++k)
+      if (!(sti[k] & HSRQ_CONVERGED)) t.wr.push_back(wr_h[k]);
+    std::copy(st_new.begin(), st_new.begin() + nc, sti);
+    std::copy(wr_new.begin(), wr_new.begin() + nc, wr_h.begin());
+    std::copy(t.wr.begin(), t.wr.end(), wr_h.begin() + nc);
+    HIP_TRY(hipMemcpyAsync(q.wr, wr_h.data(), q.m * sizeof(double), hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    t.its_pass.assign(t.cnt, INT_MAX);
+    t.stat.assign(t.cnt, 0);
+    for (int j = 0; j < t.cnt; ++j) t.pend.push_back(j);
+    tails.push_back(std::move(t));
+  }
+  if (tails.empty()) return HSRQ_OK;
+
+  // solves per round: about 2 GiB of workspace and vectors (~40 n bytes per
+  // solve), 1024 .. 16384
+  int nmax = 0;
+  size_t pend_total = 0;
+  for (const Tail& t : tails) {
+    nmax = std::max(nmax, probs[t.prob].n);
+    pend_total += t.cnt;
+  }
+  const size_t budget = std::min<size_t>(16384, std::max<size_t>(1024, ((size_t)2 << 30) / (40 * (size_t)nmax)));
+  // scratch: kept vectors, and the round buffers sized for `budget` solves
+  size_t sc = 0;
+  for (const Tail& t : tails) sc += align_up((size_t)probs[t.prob].n * t.cnt * sizeof(double));
+  const size_t round_bytes = std::max(budget, pend_total) * (size_t)nmax * sizeof(double) +
+                             tails.size() * 2 * kAlign +
+                             std::max(budget, pend_total) * (sizeof(double) + sizeof(int));
+  char* scratch = nullptr;
+  HIP_TRY(hipMalloc(&scratch, sc + align_up(round_bytes) + 2 * align_up(pend_total * 8) + kAlign));
+  DevArena ar;
+  size_t pos = 0;
+  for (Tail& t : tails) {
+    t.dbuf = (double*)(scratch + pos);
+    pos += align_up((size_t)probs[t.prob].n * t.cnt * sizeof(double));
+  }
+  char* rb = scratch + pos;
+  const size_t rb_cap = align_up(round_bytes) + 2 * align_up(pend_total * 8) + kAlign;
+  // one round; returns 1 when nothing is pending
+  auto round = [&]() -> int {
+    size_t npend = 0;
+    for (const Tail& t : tails) npend += t.pend.size();
+    if (npend == 0) return 1;
+    const int K = (int)std::max<size_t>(1, budget / npend);
+    size_t total = 0;
+    for (Tail& t : tails) {
+      t.K = t.pend.empty() ? 0 : std::min(K, probs[t.prob].n - t.its0);
+      t.mr = t.K * (int)t.pend.size();
+      total += t.mr;
+    }
+    // per tail K copies of its pending shifts, start vectors its0 .. its0+K-1
+    std::vector<double> h_wr(total);
+    std::vector<int> h_its(total);
+    double* dwr_all = (double*)rb;
+    int* dits_all = (int*)(rb + align_up(total * sizeof(double)));
+    char* pw = rb + align_up(total * sizeof(double)) + align_up(total * sizeof(int));
+    std::vector<hsrq_problem> vq;
+    std::vector<const int*> itsk;
+    std::vector<int> vtile;
+    size_t p2 = 0;
+    for (Tail& t : tails) {
+      if (t.mr == 0) continue;
+      const hsrq_problem& q = probs[t.prob];
+      t.dwr = dwr_all + p2;
+      t.dits = dits_all + p2;
+      t.dy = (double*)pw;
+      pw += align_up((size_t)q.n * t.mr * sizeof(double));
+      const size_t np = t.pend.size();
+      for (int k = 0; k < t.K; ++k)
+        for (size_t a = 0; a < np; ++a) {
+          h_wr[p2 + k * np + a] = t.wr[t.pend[a]];
+          h_its[p2 + k * np + a] = t.its0 + k;
+        }
+      p2 += t.mr;
+      vq.push_back({q.n, q.H, q.ldH, t.mr, t.dwr, t.dy, q.n});
+      itsk.push_back(t.dits);
+      vtile.push_back(tile_of[t.prob]);
+    }
+    if ((size_t)(pw - rb) > rb_cap) return HSRQ_ENOMEM;   // (the round fits by construction)
+    // a plan's problems share one tile (assign_lanes), so the rounds' do too
+    const int tl = vtile[0];
+    for (int x : vtile)
+      if (x != tl) return HSRQ_EUNSUPPORTED;
+    HIP_TRY(hipMemcpyAsync(dwr_all, h_wr.data(), total * sizeof(double), hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipMemcpyAsync(dits_all, h_its.data(), total * sizeof(int), hipMemcpyHostToDevice,
+                           stream));
+    hsrq_plan* pl = nullptr;
+    int rc2;
+    for (;;) {
+      ar.used = 0;
+      rc2 = ar.base ? plan_build(ctx, (int)vq.size(), vq.data(), tl, -1, &pl, itsk.data(), &ar)
+                    : HSRQ_ENOMEM;
+      if (rc2 != HSRQ_ENOMEM) break;
+      // grow the arena, build again
+      const size_t want = std::max<size_t>(2 * ar.cap, (size_t)64 << 20);
+      if (ar.base) (void)hipFree(ar.base);
+      ar.base = nullptr;
+      ar.cap = 0;
+      HIP_TRY(hipMalloc(&ar.base, want));
+      ar.cap = want;
+    }
+    if (rc2) return rc2;
+    rc2 = plan_enqueue(pl);
+    std::vector<int> sv(std::max<size_t>(total, 1));
+    if (!rc2) rc2 = plan_read_status(pl, sv.data());
+    plan_free(pl);
+    if (rc2) return rc2;
+    // each pending shift keeps the first of its K solves that converged, or
+    // the solve of round n-1
+    size_t base = 0;
+    for (Tail& t : tails) {
+      if (t.mr == 0) continue;
+      const hsrq_problem& q = probs[t.prob];
+      const int np = (int)t.pend.size();
+      const bool last = t.its0 + t.K >= q.n;
+      std::vector<int> keep;
+      for (int a = 0; a < np; ++a) {
+        const int j = t.pend[a];
+        int got = -1;
+        for (int k = 0; k < t.K && got < 0; ++k)
+          if (sv[base + (size_t)k * np + a] & HSRQ_CONVERGED) got = k;
+        if (got < 0 && !last) {
+          keep.push_back(j);
+          continue;
+        }
+        const int k = got >= 0 ? got : t.K - 1;
+        t.its_pass[j] = got >= 0 ? t.its0 + k : INT_MAX;
+        t.stat[j] = sv[base + (size_t)k * np + a];
+        HIP_TRY(hipMemcpyAsync(t.dbuf + (size_t)j * q.n, t.dy + ((size_t)k * np + a) * q.n,
+                               q.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      }
+      t.pend = keep;
+      t.its0 += t.K;
+      base += t.mr;
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    return 0;
+  };
+  int rc = HSRQ_OK;
+  for (;;) {
+    const int r = round();
+    if (r == 1) break;
+    if (r) {
+      rc = r;
+      break;
+    }
+  }
+  // final order of each tail: by the round of convergence, stable
+  for (Tail& t : tails) {
+    if (rc) break;
+    const hsrq_problem& q = probs[t.prob];
+    std::vector<int> order(t.cnt);
+    for (int j = 0; j < t.cnt; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return t.its_pass[a] < t.its_pass[b]; });
+    std::vector<double> wr_t(t.cnt);
+    int* sti = st.data() + off[t.prob];
+    for (int r = 0; r < t.cnt && !rc; ++r) {
+      const int j = order[r];
+      wr_t[r] = t.wr[j];
+      sti[t.first + r] = t.stat[j];
+      if (hipMemcpyAsync(q.vr + (size_t)(t.first + r) * q.ldvr, t.dbuf + (size_t)j * q.n,
+                         q.n * sizeof(double), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+        rc = HSRQ_EHIP;
+    }
+    if (!rc && hipMemcpyAsync(q.wr + t.first, wr_t.data(), t.cnt * sizeof(double),
+                              hipMemcpyHostToDevice, stream) != hipSuccess)
+      rc = HSRQ_EHIP;
+    if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = HSRQ_EHIP;
+  }
+  (void)hipStreamSynchronize(stream);
+  if (ar.base) (void)hipFree(ar.base);
+  (void)hipFree(scratch);
+  return rc;
+}
+
+// Solves a set of problems: one plan for all of them, then the restarts of
+// the shifts that did not converge (restart_rounds).
 int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz,
                         int* status_out) {
   hsrq_plan* p = nullptr;
@@ -529,59 +802,8 @@ int solve_with_restarts(hsrq_context* ctx, int count, const hsrq_problem* probs,
   std::vector<int> tile_of(p->tile_of);
   plan_free(p);
   if (rc) return rc;
-
-  for (int i = 0; i < count; ++i) {
-    hsrq_problem q = probs[i];
-    int* sti = st.data() + off[i];
-    int first = 0;
-    for (int its = 0; its < q.n; ++its) {
-      const int cnt = q.m - first;
-      int nc = 0;
-      for (int r = first; r < q.m; ++r) nc += (sti[r] & HSRQ_CONVERGED) ? 1 : 0;
-      if (nc == cnt) break;
-      // stable partition of [first, m): converged first
-      std::vector<int> order;
-      order.reserve(cnt);
-      for (int r = first; r < q.m; ++r)
-        if (sti[r] & HSRQ_CONVERGED) order.push_back(r);
-      for (int r = first; r < q.m; ++r)
-        if (!(sti[r] & HSRQ_CONVERGED)) order.push_back(r);
-      std::vector<double> wr_h(q.m);
-      HIP_TRY(hipMemcpy(wr_h.data(), q.wr, q.m * sizeof(double), hipMemcpyDeviceToHost));
-      double* tmp = nullptr;
-      HIP_TRY(hipMalloc(&tmp, (size_t)q.n * cnt * sizeof(double)));
-      std::vector<double> wr_new(wr_h);
-      std::vector<int> st_new(sti, sti + q.m);
-      for (int t = 0; t < cnt; ++t) {
-        const int src = order[t];
-        wr_new[first + t] = wr_h[src];
-        st_new[first + t] = sti[src];
-        HIP_TRY(hipMemcpyAsync(tmp + (size_t)t * q.n, q.vr + (size_t)src * q.ldvr,
-                               q.n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      }
-      HIP_TRY(hipMemcpy2DAsync(q.vr + (size_t)first * q.ldvr, q.ldvr * sizeof(double), tmp,
-                               q.n * sizeof(double), q.n * sizeof(double), cnt,
-                               hipMemcpyDeviceToDevice, ctx->stream));
-      HIP_TRY(hipMemcpyAsync(q.wr, wr_new.data(), q.m * sizeof(double), hipMemcpyHostToDevice,
-                             ctx->stream));
-      HIP_TRY(hipStreamSynchronize(ctx->stream));
-      (void)hipFree(tmp);
-      std::copy(st_new.begin(), st_new.end(), sti);
-      first += nc;
-      // re-solve the unconverged tail with the its-th restart vector
-      hsrq_problem tail = q;
-      tail.m = q.m - first;
-      tail.wr = q.wr + first;
-      tail.vr = q.vr + (size_t)first * q.ldvr;
-      hsrq_plan* pt = nullptr;
-      rc = plan_build(ctx, 1, &tail, tile_of[i], its, &pt);
-      if (rc) return rc;
-      rc = plan_enqueue(pt);
-      if (!rc) rc = plan_read_status(pt, sti + first);
-      plan_free(pt);
-      if (rc) return rc;
-    }
-  }
+  rc = restart_rounds(ctx, count, probs, tile_of, off, st);
+  if (rc) return rc;
   if (status_out) std::copy(st.begin(), st.begin() + off[count], status_out);
   return HSRQ_OK;
 }

@@ -69,9 +69,11 @@ int lookahead_blocks_per_cu(int pack, int b);
 
 // Device workspace of one problem (bytes, 256-aligned pieces).
 size_t prob_workspace_bytes(int n, int m, int b);
-// Fills the descriptor of problem q with tile b, start vector `its` and the
-// workspace at `ws` (prob_workspace_bytes(q.n, q.m, b) bytes).
-void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws);
+// Fills the descriptor of problem q with tile b, start vector `its` (or per
+// shift its_k, device, m entries) and the workspace at `ws`
+// (prob_workspace_bytes(q.n, q.m, b) bytes).
+void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws,
+                   const int* its_k = nullptr);
 
 // Item tables.  `i` is the problem's index in the descriptor array.  The
 // builders append to any container with push_back (std::vector, or the

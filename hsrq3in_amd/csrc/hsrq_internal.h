@@ -45,6 +45,8 @@ struct ProbDev {
   int* status;        // per shift: bit0 converged, bit1 perturbed pivot
   int its;            // -1: start vector eps3*1; >= 0: dlaein restart vector;
                       // -2: the right-hand sides are given in Y
+  const int* its_k;   // if set: the start vector of shift k is its_k[k] (-1 or
+                      // >= 0 as its), overriding its
   int force_exact;    // 1: always use exact norms in the diagonal sweep
 };
 

@@ -130,18 +130,19 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
   rowblock_range(P, it.rb, tile, r0, nr);
   const int n = P.n;
   double yv = 0.0;
+  const int its = P.its_k ? P.its_k[k] : P.its;
   if (lane < nr) {
     const int row = r0 + lane;
-    if (P.its == -2) {
+    if (its == -2) {
       yv = P.Y[row + (int64_t)k * P.ldY];   // given right-hand side
-    } else if (P.its < 0) {
+    } else if (its < 0) {
       yv = eps3;
     } else {
       const double sq = sqrt((double)n);
       yv = (row == 0) ? eps3 : eps3 / (sq + 1.0);
-      if (row == n - 1 - P.its) yv -= eps3 * sq;
+      if (row == n - 1 - its) yv -= eps3 * sq;
     }
-    if (P.its != -2) P.Y[row + (int64_t)k * P.ldY] = yv;
+    if (its != -2) P.Y[row + (int64_t)k * P.ldY] = yv;
     double h = P.H[row + (int64_t)(n - 1) * P.ldH];
     if (row == n - 1) h -= P.wr[k];
     P.Rt[(P.T - 1) & 1][row + (int64_t)k * n] = h;

@@ -96,7 +96,8 @@ Lay layout(int n, int m, int b) {
 
 size_t prob_workspace_bytes(int n, int m, int b) { return layout(n, m, b).total; }
 
-void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws) {
+void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws,
+                   const int* its_k) {
   std::memset(&P, 0, sizeof(P));
   P.H = q.H;
   P.ldH = q.ldH;
@@ -111,6 +112,7 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.Y = q.vr;
   P.ldY = q.ldvr;
   P.its = its;
+  P.its_k = its_k;
   P.force_exact = exact;
   const Lay L = layout(q.n, q.m, b);
   P.Rt[0] = (double*)(ws + L.rt0);

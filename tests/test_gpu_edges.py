@@ -137,3 +137,47 @@ def test_dropin_padded_host_buffers():
     c.close()
     assert np.array_equal(wr_io, w0)
     assert np.array_equal(vr[:n, :], v0.T)
+
+
+@pytest.mark.parametrize("n,m,nc", [(300, 40, 12), (500, 60, 20)])
+def test_many_restarts_against_oracle_and_sequential(ctx, n, m, nc):
+    """A clustered spectrum whose first solve converges only 4 of the shifts:
+    the other shifts need hundreds of restart rounds (the last at round n-1).
+    The restart engine evaluates many rounds per launch; its vectors, shift
+    order and status equal the one-round-at-a-time loop of the allocation-free
+    entry point bit for bit, and the oracle's restart loop
+    (inverse_iteration.c:160-328) to the bar (1e-10; cluster members 1e-8,
+    their eigenvectors move by eps ||H|| / spacing)."""
+    import ctypes
+
+    from hsrq3in_amd import lib
+    from hsrq3in_amd import workloads as w
+    lam, shifts = w.cluster_spectrum(n, m, nc, 4)
+    H = w.cluster_hessenberg(lam, nc, 4)
+    v, wo, st = gpu_solve(ctx, H, shifts, 128)
+    # the sequential loop (hsrq_solve_device_noalloc restarts one round at a time)
+    Ht = colmajor_to_device(H)
+    wrt = torch.from_numpy(shifts.copy()).cuda()
+    vrt = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    st2 = np.zeros(m, dtype=np.int32)
+    L = lib()
+    torch.cuda.synchronize()
+    assert L.hsrq_solve_device_noalloc(ctx.handle, n, Ht.data_ptr(), n, m, wrt.data_ptr(),
+                                       vrt.data_ptr(), n, 128,
+                                       st2.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) == 0
+    assert np.array_equal(st, st2)
+    assert np.array_equal(wo, wrt.cpu().numpy())
+    assert np.array_equal(v, vrt.cpu().numpy())
+    # the oracle
+    xo, woo, sto, so, lsc = oracle.tiled_inverse_iteration_detail(H, shifts, 128, 16)
+    assert sto.restarts >= n - 2
+    assert np.array_equal(wo, woo)
+    assert np.array_equal(st & (HSRQ_CONVERGED | HSRQ_PERTURBED), so & 3)
+    x = v.T
+    conv = (st & HSRQ_CONVERGED) != 0
+    d = 1 - np.abs(np.sum(x[:, conv] * xo[:, conv], axis=0))
+    member = np.isin(wo[conv], shifts[:nc])
+    print(f"n={n}: {conv.sum()}/{m} converged after {sto.restarts} rounds; 1-|cos| cluster "
+          f"{d[member].max() if member.any() else 0:.2e}, others "
+          f"{d[~member].max() if (~member).any() else 0:.2e}")
+    assert (d[~member] <= 1e-10).all() and (d[member] <= 1e-8).all()

@@ -74,14 +74,49 @@ def sweep_bytes(n: int, m: int, b: int, shifts_per_group: int) -> float:
     return tot
 
 
-def cpu_baseline(H, shifts, tile, threads):
+def cpu_baseline(H, shifts, tile, threads, reps=5):
+    """The oracle (the reference's algorithm restated in C/OpenMP) on the host
+    cores: one warm-up solve, then the median of `reps` timed solves."""
     import oracle
-    t0 = time.perf_counter()
-    _, _, st = oracle.tiled_inverse_iteration(H, shifts, tile, 16, threads)
-    dt = time.perf_counter() - t0
+    oracle.tiled_inverse_iteration(H, shifts, tile, 16, threads)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.tiled_inverse_iteration(H, shifts, tile, 16, threads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return {"value": len(shifts) / dt, "unit": "eigvecs/s", "cores": threads, "kind": "port",
             "sample": f"all {len(shifts)} shifts of the same n={H.shape[0]} problem, tile {tile}, "
-                      f"shift tile 16, {dt:.2f} s wall"}
+                      f"shift tile 16: median of {reps} solves after a warm-up "
+                      f"({dt:.2f} s; min {min(times):.2f}, max {max(times):.2f} s)"}
+
+
+def dropin_pcie(H, shifts, tile, reps=3):
+    """The drop-in entry point on host buffers (tiled_inverse_iteration), timed
+    from call entry to return as the reference's driver times it
+    (main.c:101-103): H (512 MB at n = 8000) and wr in, vr out over PCIe from
+    pageable memory, plus the solve with its restarts.  Median of `reps`
+    after a warm-up call."""
+    import ctypes
+    from hsrq3in_amd import lib
+    n, m = H.shape[0], len(shifts)
+    Hf = np.asfortranarray(H)
+    vr = np.zeros((n, m), order="F")
+    dp = ctypes.POINTER(ctypes.c_double)
+    times = []
+    for r in range(reps + 1):
+        wr = np.ascontiguousarray(shifts, dtype=np.float64).copy()
+        t0 = time.perf_counter()
+        lib().tiled_inverse_iteration(n, Hf.ctypes.data_as(dp), n, m, wr.ctypes.data_as(dp),
+                                      tile, 16, vr.ctypes.data_as(dp), n)
+        if r:
+            times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    return {"value": round(m / dt, 1), "unit": "eigvecs/s", "ms": round(1e3 * dt, 2),
+            "host_to_device_MB": round(8 * (n * n + m) / 1e6, 1),
+            "device_to_host_MB": round(8 * (n * m + m) / 1e6, 1),
+            "timing": "tiled_inverse_iteration call on host buffers, entry to return "
+                      "(PCIe-inclusive, pageable memory), median of 3 after a warm-up"}
 
 
 def batched(ctx, steps, warmup, tile, plain_sample):
@@ -117,9 +152,11 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     st = plan.status()
     good, pos, worst = 0, 0, 0.0
-    for Ht, wrt, vrt in items:
+    t0 = time.perf_counter()
+    allres = ctx.residuals_batch(items)   # every vector, one device pass
+    val_ms = 1e3 * (time.perf_counter() - t0)
+    for (Ht, wrt, vrt), res in zip(items, allres):
         k = wrt.numel()
-        res = ctx.residuals(Ht, wrt, vrt)
         n = Ht.shape[0]
         ok = (res <= 10 * n) & ((st[pos:pos + k] & HSRQ_CONVERGED) != 0)
         good += int(ok.sum())
@@ -147,6 +184,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     return {"value": round(batched_rate, 1), "unit": "eigvecs/s", "problems": len(items),
             "eigvecs": total_m, "residual_ok": good, "finite": finite,
             "worst_residual_over_bound": worst, "ms_per_batch": round(ms, 3),
+            "validation_ms": round(val_ms, 2),
             "lanes": [{"tile": t, "shifts_per_wave": pk, "problems": pr} for t, pk, pr in lanes],
             "autotuned": tuned,
             "plain_one_at_a_time": {"value": round(plain_rate, 1), "problems": len(sample),
@@ -199,7 +237,17 @@ def around_path(ctx, reps=3):
     V = torch.empty_like(X)
     bt_ms = timed(lambda: L.hsrq_backtransform_device(ctx.handle, n, m, Ht.data_ptr(), n,
                                                       X.data_ptr(), n, V.data_ptr(), n))
+    wr8 = torch.from_numpy(np.linspace(-3.0, 3.0, m)).cuda()
+    ctx.residuals(Ht, wr8, X)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.residuals(Ht, wr8, X)
+    val_ms = 1e3 * (time.perf_counter() - t0) / reps
     return {
+        "validator": {"n": n, "m": m, "ms": round(val_ms, 3),
+                      "TFLOPs": round(1.0 * n * n * m / val_ms / 1e9, 1),
+                      "timing": "wall clock of hsrq_residuals_device (synchronous, incl. its "
+                                "workspace allocation); Hessenberg product n^2 m flops"},
         "gaussian_generator": {"n": n, "ms": round(gen_ms, 3),
                                "written_GBs": round(8 * n * n / gen_ms / 1e6, 1)},
         "complex_shifts": {"n": n, "m": mc, "ms": round(cplx_ms, 2),
@@ -264,10 +312,28 @@ def plain_variant(ctx, kind, steps, tile):
         plan.close()
     res = ctx.residuals(Ht, wrt, vrt)
     good = int(np.sum((res <= 10 * n) & ((st & HSRQ_CONVERGED) != 0)))
-    return {"value": round(good / (ms / 1e3), 1), "unit": "eigvecs/s", "ms_per_step": round(ms, 4),
-            "residual_ok": good, "converged": int(np.sum((st & HSRQ_CONVERGED) != 0)),
-            "eigvecs": m, "residual_max": float(np.max(res)),
-            "finite": bool(torch.isfinite(vrt).all().item()), "path": path}
+    out = {"value": round(good / (ms / 1e3), 1), "unit": "eigvecs/s", "ms_per_step": round(ms, 4),
+           "residual_ok": good, "converged": int(np.sum((st & HSRQ_CONVERGED) != 0)),
+           "eigvecs": m, "residual_max": float(np.max(res)),
+           "finite": bool(torch.isfinite(vrt).all().item()), "path": path}
+    if kind != "graded":
+        # the same problem solved in full: the shifts the first pass leaves
+        # get the reference's restart vectors (up to ~7700 rounds here),
+        # evaluated many rounds per launch; one timed solve
+        wrt.copy_(torch.from_numpy(shifts))
+        torch.cuda.synchronize()
+        print("[bench] variant cluster with restarts", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        st = ctx.solve(Ht, wrt, vrt, tile)
+        dt = time.perf_counter() - t0
+        res = ctx.residuals(Ht, wrt, vrt)
+        good = int(np.sum((res <= 10 * n) & ((st & HSRQ_CONVERGED) != 0)))
+        out["with_restarts"] = {"value": round(good / dt, 1), "unit": "eigvecs/s",
+                                "seconds": round(dt, 3), "residual_ok": good,
+                                "converged": int(np.sum((st & HSRQ_CONVERGED) != 0)),
+                                "residual_max": float(np.max(res)),
+                                "path": "hsrq_solve_device (first pass + restart rounds)"}
+    return out
 
 
 def precision_study(n=20000, m=2000, frac=0.1, kmax=20, seed=11):
@@ -332,21 +398,23 @@ def precision_study(n=20000, m=2000, frac=0.1, kmax=20, seed=11):
             "shifts_with_relative_gap_ge_1e-6": int(sep.sum()), "modes": out}
 
 
-def served(ctx, qps, nreq, pool_size, seed=9):
+def served(ctx, qps_list, nreq, pool_size, seed=9):
     """Serving workload: Poisson arrivals at a fixed rate of requests with n
     log-uniform in [500, 16000], m uniform in [1, n/10] shifts, 15 % graded,
     through the continuous-batching server; H resident in HBM (a pool of
-    problems uploaded once), X and the scale exponents kept in HBM."""
+    problems built once on the device), X and the scale exponents kept in
+    HBM, every result validated online by the server.  One replay per rate
+    of qps_list (a sweep to saturation); the first rate is the headline."""
     import threading
     import torch
-    from hsrq3in_amd import HSRQ_CONVERGED, colmajor_to_device, workloads
+    from hsrq3in_amd import HSRQ_CONVERGED, HSRQ_VALIDATED, workloads
     from hsrq3in_amd.serve import Server
     t0 = time.perf_counter()
     pool = []
     for n, g, sd in workloads.serving_sizes(pool_size):
-        H, lam = workloads.serving_problem(n, sd, g)
-        pool.append((colmajor_to_device(H), lam, n, g))
-        del H
+        Ht, lam = workloads.serving_problem_device(ctx, n, sd, g)
+        pool.append((Ht, lam, n, g))
+    torch.cuda.synchronize()
     print(f"[bench] serving pool of {len(pool)} problems built in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
     rng = np.random.default_rng(seed)
@@ -356,52 +424,59 @@ def served(ctx, qps, nreq, pool_size, seed=9):
         n = pool[p][2]
         m = int(rng.integers(1, n // 10 + 1))
         sh = rng.choice(pool[p][1], size=m, replace=False)
-        reqs.append((p, torch.from_numpy(np.ascontiguousarray(sh)).cuda(),
-                     torch.zeros((m, n), dtype=torch.float64, device="cuda")))
-    gaps = rng.exponential(1.0 / qps, size=nreq)
-    torch.cuda.synchronize()
-    srv = Server(ctx, hbm_bytes=96 << 30, max_active=512)
-    ids = [None] * nreq
+        reqs.append((p, np.ascontiguousarray(sh), torch.zeros((m, n), dtype=torch.float64,
+                                                              device="cuda")))
+    sweep = []
+    for qps in qps_list:
+        gaps = np.random.default_rng(seed + 1).exponential(1.0 / qps, size=nreq)
+        wrts = [torch.from_numpy(sh.copy()).cuda() for _, sh, _ in reqs]
+        torch.cuda.synchronize()
+        srv = Server(ctx, hbm_bytes=96 << 30, max_active=512)
+        ids = [None] * nreq
 
-    def producer():
-        t_next = time.perf_counter()
-        for i, (p, wrt, vrt) in enumerate(reqs):
-            t_next += gaps[i]
-            dt = t_next - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            ids[i] = srv.submit_device(pool[p][0], wrt, vrt)
+        def producer():
+            t_next = time.perf_counter()
+            for i, (p, _, vrt) in enumerate(reqs):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit_device(pool[p][0], wrts[i], vrt)
 
-    th = threading.Thread(target=producer)
-    t0 = time.perf_counter()
-    th.start()
-    lat, good, total = [], 0, 0
-    for i in range(nreq):
-        while ids[i] is None:
-            time.sleep(0.0005)
-        _, wo, st, ms = srv.wait(ids[i])
-        lat.append(ms)
-        good += int(np.sum((st & HSRQ_CONVERGED) != 0))
-        total += st.size
-    wall = time.perf_counter() - t0
-    th.join()
-    stats = srv.stats()
-    srv.close()
-    # residuals of a sample of requests (device validator)
-    worst, finite = 0.0, True
-    for i in range(0, nreq, max(1, nreq // 24)):
-        p, wrt, vrt = reqs[i]
-        res = ctx.residuals(pool[p][0], wrt, vrt)
-        worst = max(worst, float(np.max(res)) / (10 * pool[p][2]))
-        finite = finite and bool(torch.isfinite(vrt).all().item())
-    return {"value": round(good / wall, 1), "unit": "eigvecs/s", "qps": qps, "requests": nreq,
-            "eigvecs": total, "converged": good, "wall_s": round(wall, 3),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "server_p50_ms": round(stats["p50_ms"], 3), "server_p99_ms": round(stats["p99_ms"], 3),
-            "ticks": stats["ticks"], "hbm_peak_gb": round(stats["hbm_peak"] / 2**30, 3),
-            "pool": [[n, g] for _, _, n, g in pool],
-            "sample_worst_residual_over_bound": worst, "finite": finite}
+        th = threading.Thread(target=producer)
+        t0 = time.perf_counter()
+        th.start()
+        lat, good, total, valid = [], 0, 0, 0
+        for i in range(nreq):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            _, wo, st, ms = srv.wait(ids[i])
+            lat.append(ms)
+            good += int(np.sum((st & HSRQ_CONVERGED) != 0))
+            valid += int(np.sum((st & HSRQ_VALIDATED) != 0))
+            total += st.size
+        wall = time.perf_counter() - t0
+        th.join()
+        stats = srv.stats()
+        srv.close()
+        point = {"qps": qps, "value": round(valid / wall, 1), "unit": "eigvecs/s",
+                 "requests": nreq, "eigvecs": total, "converged": good, "validated": valid,
+                 "wall_s": round(wall, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                 "worst_residual_over_bound": stats["worst_residual"],
+                 "ticks": stats["ticks"], "hbm_peak_gb": round(stats["hbm_peak"] / 2**30, 3)}
+        sweep.append(point)
+        print(f"[bench] served at {qps} req/s: {point['value']} eigvecs/s, p50 "
+              f"{point['p50_ms']} ms, p99 {point['p99_ms']} ms", file=sys.stderr, flush=True)
+    finite = all(bool(torch.isfinite(v).all().item()) for _, _, v in reqs)
+    out = dict(sweep[0])
+    out.update({"pool": [[n, g] for _, _, n, g in pool], "finite": finite,
+                "validation": "every result, online by the server (HSRQ_VALIDATED: "
+                              "residual <= 10 n eps); value counts validated vectors",
+                "qps_sweep": sweep,
+                "saturation": {"value": max(p["value"] for p in sweep),
+                               "qps": max(sweep, key=lambda p: p["value"])["qps"]}})
+    return out
 
 
 def main():
@@ -427,8 +502,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="bytes: run one elementwise copy of this size (counter calibration)")
-    ap.add_argument("--serve-qps", type=float, default=100.0)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps", type=str, default="100,25,50,200,400,800,1600",
+                    help="request rates of the serving sweep (the first is the headline)")
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-pool", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -538,7 +614,11 @@ def main():
         out["precision"] = {"graded": precision_study(), "ungraded": precision_study(frac=0.0)}
     if not args.no_serve:
         ctx.reset_tuning()
-        out["served"] = served(ctx, args.serve_qps, args.serve_requests, args.serve_pool)
+        out["served"] = served(ctx, [float(q) for q in args.serve_qps.split(",")],
+                               args.serve_requests, args.serve_pool)
+    if (n, m) == (8000, 800) and not args.no_variants:
+        print("[bench] drop-in entry point on host buffers", file=sys.stderr, flush=True)
+        out["dropin_pcie"] = dropin_pcie(H, shifts, tile)
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         threads = min(16, os.cpu_count() or 1)

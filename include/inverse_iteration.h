@@ -8,8 +8,10 @@
  *   overflow protection), with restarts for unconverged shifts.
  *
  * @param n        Order of H, n > 0.
- * @param H        Host pointer, column-major, ldH >= n.  Entries below the
- *                 subdiagonal must be finite (they enter ||H||_oo).
+ * @param H        Host pointer, column-major, ldH >= n.  Only the upper
+ *                 Hessenberg part is used (and copied to the device, in
+ *                 column panels); entries below the subdiagonal do not enter
+ *                 the result.
  * @param nrhs     Number of shifts.
  * @param wr       Host pointer, nrhs shifts; may be reordered on return
  *                 (converged first) when restarts were needed.

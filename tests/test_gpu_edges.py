@@ -181,3 +181,17 @@ def test_many_restarts_against_oracle_and_sequential(ctx, n, m, nc):
           f"{d[member].max() if member.any() else 0:.2e}, others "
           f"{d[~member].max() if (~member).any() else 0:.2e}")
     assert (d[~member] <= 1e-10).all() and (d[member] <= 1e-8).all()
+
+
+@pytest.mark.parametrize("n,tile", [(300, 64), (700, 128), (600, 256)])
+def test_below_subdiagonal_never_read(ctx, n, tile):
+    """NaN in every entry below the subdiagonal: the solve reads only the
+    Hessenberg part (results bit for bit those of the zero-filled matrix), so
+    the drop-in's Hessenberg-only H2D copy is exact."""
+    H, wr = case(n, seed=n + 77, m=16, kmax=30)
+    v0, w0, s0 = gpu_solve(ctx, H, wr, tile)
+    Hn = H.copy(order="F")
+    i, j = np.tril_indices(n, -2)
+    Hn[i, j] = np.nan
+    v1, w1, s1 = gpu_solve(ctx, Hn, wr, tile)
+    assert np.array_equal(v0, v1) and np.array_equal(w0, w1) and np.array_equal(s0, s1)

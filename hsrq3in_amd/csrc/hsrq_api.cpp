@@ -920,73 +920,10 @@ int solve_reduced(hsrq_context* ctx, int count, const hsrq_problem* probs, int t
   return HSRQ_OK;
 }
 
-// Row-norm spread max_i / min_i ||H(i,:)||_1 above which HSRQ_PREC_FP32
-// keeps a problem on the FP64 path: on graded matrices the FP32 panel pass
-// loses the small components of the vectors, and the residual relative to
-// ||H||_oo does not see it.
-constexpr double kFp32MaxRowSpread = 0x1p20;
-
-int row_spread(hsrq_context* ctx, const hsrq_problem& q, double* spread) {
-  const int n = q.n;
-  double* work = nullptr;
-  HIP_TRY(hipMalloc(&work, 65 * (size_t)n * sizeof(double)));
-  std::vector<double> rs(n);
-  hipError_t e = launch_rowsums(n, q.H, q.ldH, work + n, work, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(rs.data(), work, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(work);
-  if (e != hipSuccess) {
-    last_hip_error() = e;
-    return HSRQ_EHIP;
-  }
-  double lo = INFINITY, hi = 0.0;
-  for (double v : rs) {
-    lo = std::min(lo, v);
-    hi = std::max(hi, v);
-  }
-  *spread = lo > 0.0 ? hi / lo : INFINITY;
-  return HSRQ_OK;
-}
-
 int solve_any(hsrq_context* ctx, int count, const hsrq_problem* probs, int tlsz, int* status) {
   if (ctx->precision == HSRQ_PREC_FP64)
     return solve_with_restarts(ctx, count, probs, tlsz, status);
-  if (ctx->precision == HSRQ_PREC_FP32_RAW) return solve_reduced(ctx, count, probs, tlsz, status);
-  // HSRQ_PREC_FP32: graded problems go the FP64 way
-  std::vector<int> fp32, fp64;
-  for (int i = 0; i < count; ++i) {
-    double spread = 0.0;
-    const int rc = row_spread(ctx, probs[i], &spread);
-    if (rc) return rc;
-    (spread <= kFp32MaxRowSpread ? fp32 : fp64).push_back(i);
-  }
-  if (fp64.empty()) return solve_reduced(ctx, count, probs, tlsz, status);
-  std::vector<int> off(count + 1, 0);
-  for (int i = 0; i < count; ++i) off[i + 1] = off[i] + probs[i].m;
-  for (int pass = 0; pass < 2; ++pass) {
-    const std::vector<int>& idx = pass == 0 ? fp32 : fp64;
-    if (idx.empty()) continue;
-    std::vector<hsrq_problem> sub;
-    for (int i : idx) sub.push_back(probs[i]);
-    int tot = 0;
-    for (const auto& q : sub) tot += q.m;
-    std::vector<int> st(std::max(tot, 1));
-    const int prec = ctx->precision;
-    if (pass == 1) ctx->precision = HSRQ_PREC_FP64;
-    const int rc = pass == 0 ? solve_reduced(ctx, (int)sub.size(), sub.data(), tlsz, st.data())
-                             : solve_with_restarts(ctx, (int)sub.size(), sub.data(), tlsz, st.data());
-    ctx->precision = prec;
-    if (rc) return rc;
-    if (status) {
-      int pos = 0;
-      for (int i : idx) {
-        std::copy(st.begin() + pos, st.begin() + pos + probs[i].m, status + off[i]);
-        pos += probs[i].m;
-      }
-    }
-  }
-  return HSRQ_OK;
+  return solve_reduced(ctx, count, probs, tlsz, status);
 }
 
 }  // namespace

@@ -1509,10 +1509,12 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 // ---------------------------------------------------------------------------
 // Panel pass with FP32 MFMA (v_mfma_f32_16x16x4_f32) and FP32 accumulation:
 // the optional reduced-precision update path.  Same blocking and epilogue as
-// k_update; H is staged as float scaled by 2^-eH (||H||_oo <= 2^(eH+1), so
-// the scaled entries are at most 2), [W | Z] as float (|W| <= 1, Z carries
-// its prescale), and the FP32 result is scaled back and finished in FP64
-// (cross-over terms, shift correction, scaled update of y).  FP32 MFMA
+// k_update.  The block's panel and [W | Z] columns are equilibrated by powers
+// of two before the conversion to float: H(i, j) 2^(c_j - r_i) (column
+// exponents over the block's rows, then row exponents) and B(j, s)
+// 2^(-c_j - e_s), so graded matrices (entries 2^+-800 and more) keep their
+// small terms; the FP32 result is scaled back by 2^(r_i + e_s) and finished
+// in FP64 (cross-over terms, shift correction, scaled update of y).  FP32 MFMA
 // layout: D[i][j] in register i%4 of lane j + 16 (i/4); here D = D^T[col][row],
 // so a lane holds one row and four adjacent columns, W (even) and Z (odd).
 // ---------------------------------------------------------------------------
@@ -1522,8 +1524,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
   __shared__ float As[2][KC * LDA];
   __shared__ float Bs[2][KC * LDB];
   __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
-  __shared__ int psy[GN / 2], psz[GN / 2], pez[GN / 2];
+  __shared__ int psy[GN / 2], psz[GN / 2];
   __shared__ double red[2][GN];
+  // power-of-two equilibration of the block: column exponents of the panel
+  // (b <= 256 columns), row exponents of the column-scaled panel, exponents
+  // of the [W | Z] columns scaled by the inverse column exponents
+  __shared__ int ccol[256], rrow[GM], ecol[GN];
+  __shared__ double rmx[4][GM];
 
   // LOOP: grid-stride over the items, so that a launch beside a diagonal
   // sweep can be capped to leave room for it on every CU; otherwise one
@@ -1539,7 +1546,6 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     rowblock_range(P, it.rb, tile, r0, nr);
     const int col0 = it.cb * GN;
     const int ncols = min(GN, 2 * m - col0);
-    const int eH = P.scal[0] > 0.0 ? ilogb(P.scal[0]) : 0;
 
     // the thread index is re-read per item (an empty asm makes it opaque), so
     // that the index arithmetic derived from it is not held in registers
@@ -1560,15 +1566,17 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
-        pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH], -eH) : 0.0f;
+        pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH],
+                                         ccol[kc + kk] - rrow[rr])
+                          : 0.0f;
       }
 #pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
-        // Z columns (odd) scaled to max |Z| in [1, 2) by their own power of two
-        const int ez = (cc & 1) ? pez[cc >> 1] : 0;
-        pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b], -ez) : 0.0f;
+        pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b],
+                                            -ccol[kc + k2] - ecol[cc])
+                             : 0.0f;
       }
     };
     auto stash = [&](int buf) {
@@ -1595,7 +1603,6 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     const int nchunks = b / KC;
     if (tid < GN / 2) {
       const int k = col0 / 2 + tid;
-      int ez = 0;
       if (k < m) {
         const double* aux = P.aux[J & 1];
         const double mu = P.wr[k];
@@ -1609,10 +1616,38 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
         psy[tid] = sy;
         psz[tid] = sz;
         if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
-        const double zn = aux[4 * k + 2];   // ||Z|| before the prescale 2^-p
-        if (zn > 0.0 && isfinite(zn)) ez = ilogb(zn) - (int)aux[4 * k + 3];
       }
-      pez[tid] = ez;
+    }
+    // Equilibration (graded panels: H(i, j) 2^(c_j - r_i) and B(j, s)
+    // 2^(-c_j - e_s) are of order one wherever H = D H0 D^-1 with a diagonal
+    // D of powers of two, however wide its range; the product is scaled back
+    // by 2^(r_i + e_s) in FP64).  1: column exponents over the block's rows.
+    for (int j = wave; j < b; j += 4) {
+      const double a = wave_max(lane < nr ? fabs(Hp[(r0 + lane) + (int64_t)j * ldH]) : 0.0);
+      if (lane == 0) ccol[j] = (a > 0.0 && isfinite(a)) ? -ilogb(a) : 0;
+    }
+    __syncthreads();
+    {   // 2: row exponents of the column-scaled panel
+      const int row = tid & 63, grp = tid >> 6;
+      double a = 0.0;
+      if (row < nr)
+        for (int j = grp; j < b; j += 4)
+          a = fmax(a, ldexp(fabs(Hp[(r0 + row) + (int64_t)j * ldH]), ccol[j]));
+      rmx[grp][row] = a;
+    }
+    __syncthreads();
+    if (tid < GM) {
+      const double a = fmax(fmax(rmx[0][tid], rmx[1][tid]), fmax(rmx[2][tid], rmx[3][tid]));
+      rrow[tid] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
+    }
+    // 3: exponents of the scaled [W | Z] columns
+    for (int cc = wave * (GN / 4); cc < (wave + 1) * (GN / 4); ++cc) {
+      double a = 0.0;
+      if (cc < ncols)
+        for (int j = lane; j < b; j += 64)
+          a = fmax(a, ldexp(fabs(Bg[j + (int64_t)(col0 + cc) * b]), -ccol[j]));
+      a = wave_max(a);
+      if (lane == 0) ecol[cc] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
     }
     __syncthreads();
     fetch(0);
@@ -1682,13 +1717,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
           const double coef = isz ? pZl[ks] : pP[ks];
           const double b0 = isz ? pb0z[ks] : pb0w[ks];
           const int sy = psy[ks], sz = psz[ks];
-          const int es = eH + (isz ? pez[ks] : 0);
+          const int es = ecol[cl];
 #pragma unroll
           for (int a = 0; a < 2; ++a) {
             const int row = wm * 32 + a * 16 + rl;
             const int gi = r0 + row;
             if (row < nr) {
-              double val = fma(coef, rin[c][r][a], ldexp((double)acc[a][c][r], es));
+              double val = fma(coef, rin[c][r][a], ldexp((double)acc[a][c][r], es + rrow[row]));
               if (gi == l - 1) val -= mu * b0;
               if (!isz) {
                 Rout[gi + (int64_t)k * n] = val;

@@ -60,8 +60,9 @@ enum {
 
 /* Update precision of the panel pass. */
 #define HSRQ_PREC_FP64 0
-#define HSRQ_PREC_FP32 1     /* FP32 MFMA panel GEMM, FP64-residual check,
-                                FP64 re-solve of the vectors that fail it */
+#define HSRQ_PREC_FP32 1     /* FP32 MFMA panel GEMM (power-of-two
+                                equilibrated), FP64-residual check, FP64
+                                re-solve of the vectors that fail it */
 #define HSRQ_PREC_FP32_RAW 2 /* FP32 MFMA panel GEMM, no correction        */
 
 typedef struct hsrq_context hsrq_context;
@@ -95,14 +96,15 @@ int hsrq_context_set_lanes(hsrq_context *ctx, int lanes);
 int hsrq_context_set_lookahead(hsrq_context *ctx, int blocks_per_cu);
 /* Panel-pass precision: HSRQ_PREC_FP64 (default), HSRQ_PREC_FP32 or
  * HSRQ_PREC_FP32_RAW.  In the FP32 modes the panel GEMM runs in FP32 on MFMA
- * with FP32 accumulation; the diagonal sweeps, bounds, scale exponents,
- * back-transform and residuals stay FP64.  HSRQ_PREC_FP32 (hsrq_solve_device,
+ * with FP32 accumulation, each 64-row block of the panel and the [W | Z]
+ * columns equilibrated by powers of two first (column, then row exponents),
+ * so graded matrices keep their small terms; the diagonal sweeps (with the
+ * next tile's own-rows update), bounds, scale exponents, back-transform and
+ * residuals stay FP64.  HSRQ_PREC_FP32 (hsrq_solve_device,
  * hsrq_solve_batch_device): after the FP32 pass the FP64 residual of every
  * vector is computed, and the shifts whose vector did not converge or whose
- * residual exceeds 10 n eps are solved again on the FP64 path.  Graded
- * problems (row 1-norms spread over more than 2^20) stay on the FP64 path:
- * there the FP32 pass loses the small vector components unseen by the
- * residual.  HSRQ_PREC_FP32_RAW and plans: the FP32 pass alone. */
+ * residual exceeds 10 n eps are solved again on the FP64 path.
+ * HSRQ_PREC_FP32_RAW and plans: the FP32 pass alone. */
 int hsrq_context_set_precision(hsrq_context *ctx, int prec);
 const char *hsrq_strerror(int code);
 /* Checks the FP64 MFMA operand layout the kernels assume. */

@@ -28,8 +28,10 @@ HIP path is at 0.05 / 0.25 / 3e-8 of the spread.  The test prints them.
   continuous-batching server, H resident in HBM: every result gets the
   property checks, those of order n <= 4000 the oracle comparison.
 * Reduced precision: FP32 MFMA panel passes (HSRQ_PREC_FP32_RAW and
-  HSRQ_PREC_FP32) against the FP64 oracle at n = 8000, m = 800, the largest
-  problem whose oracle solve stays within seconds at every shift.
+  HSRQ_PREC_FP32) against the FP64 oracle at n = 8000, m = 800, plain and
+  with 10 % of the indices graded (k in [-400, 400]); and at the
+  configuration's own size, n = 20000, m = 2000 with 10 % graded indices,
+  when its shift fixture is present (tests/golden/make_shifts.py cfg4).
 """
 import threading
 import time
@@ -216,9 +218,14 @@ def test_served_poisson_replay_against_oracle(ctx):
 
 
 @pytest.mark.parametrize("prec", [PREC_FP32_RAW, PREC_FP32], ids=["fp32raw", "fp32"])
-def test_fp32_against_oracle_n8000(prec):
+@pytest.mark.parametrize("graded", [False, True], ids=["plain", "graded-rows"])
+def test_fp32_against_oracle_n8000(prec, graded):
+    # graded: 10 % of the indices with k in [-400, 400] (the reduced-precision
+    # configuration's grading; exact similarity, so the same shifts)
     from hsrq3in_amd import Context
     H = workloads.gaussian_hessenberg(8000, 2)
+    if graded:
+        H = workloads.graded(H, 400, 2, 0.1)
     wr = workloads.load_shifts("shifts_n8000_s2.npz")
     c = Context(0, precision=prec)
     try:
@@ -233,3 +240,28 @@ def test_fp32_against_oracle_n8000(prec):
                                 f"precision {prec}")
     print(f"[fp32] precision mode {prec}: max 1-|cos| vs FP64 oracle {d:.2e}, "
           f"max residual {res.max():.1f}", flush=True)
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(__import__("os").path.join(
+    workloads.GOLDEN_DIR, "shifts_n20000_s4.npz")), reason="n = 20000 shift fixture not built")
+def test_fp32_config_n20000_against_oracle():
+    """The reduced-precision configuration at its size: n = 20000, m = 2000,
+    N(0, 1) with 10 % of the indices graded (k in [-400, 400]), HSRQ_PREC_FP32
+    against the FP64 oracle (about ten seconds of host time)."""
+    from hsrq3in_amd import Context
+    H = workloads.graded(workloads.gaussian_hessenberg(20000, 4), 400, 4, 0.1)
+    wr = workloads.load_shifts("shifts_n20000_s4.npz")
+    c = Context(0, precision=PREC_FP32)
+    try:
+        Ht = colmajor_to_device(H)
+        wrt = torch.from_numpy(wr.copy()).cuda()
+        vrt = torch.zeros((wr.size, 20000), dtype=torch.float64, device="cuda")
+        st = c.solve(Ht, wrt, vrt, 128)
+        res = c.residuals(Ht, wrt, vrt)
+    finally:
+        c.close()
+    x = vrt.cpu().numpy().T
+    del vrt
+    d, nb = check_against_oracle(H, wr, x, wrt.cpu().numpy(), st, res, 128, "n=20000 fp32")
+    print(f"[fp32] n=20000 m=2000 graded rows: max 1-|cos| vs FP64 oracle {d:.2e} "
+          f"({nb} judged against their spread), max residual {res.max():.3g}", flush=True)

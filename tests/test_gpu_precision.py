@@ -3,13 +3,18 @@
 Bar, stated per case: the vectors are finite; in HSRQ_PREC_FP32 mode every
 residual ||(H - mu I) x|| / (||H||_oo ||x|| 2^-52) stays <= 10 n (vectors
 whose FP32 result fails it are solved again on the FP64 path) and 1 - |cos|
-against the FP64 path stays <= 1e-10 on Gaussian matrices and prescribed
-spectra.  Graded matrices (row norms spread over more than 2^20) are solved
-on the FP64 path in HSRQ_PREC_FP32 mode, bit-identical to HSRQ_PREC_FP64.
-The raw FP32 pass is held to the same 1 - |cos| <= 1e-10 on every kind,
-graded ones included: the rows of the next diagonal tile are updated in
-FP64 at the start of its sweep in every mode, so the FP32 error reaches the
-rotations only through the damped cross-over terms.
+against the FP64 path stays <= 1e-10, on Gaussian matrices, prescribed
+spectra and graded ones (grading on every index or on 10 % of them, k up to
++-400).  The raw FP32 pass is held to the same 1 - |cos| <= 1e-10 on every
+kind: the panel blocks are equilibrated by powers of two before the
+conversion to float (column, then row exponents; the [W | Z] columns by the
+inverse column exponents), so a graded panel keeps its small terms, and the
+rows of the next diagonal tile are updated in FP64 at the start of its sweep
+in every mode, so the FP32 error reaches the rotations only through the
+damped cross-over terms.  On graded matrices the small components are
+compared too: the FP32 vectors agree with the FP64 ones componentwise down
+to 1e-250 of the largest, at the bar the FP64 path meets against the oracle
+(tests/test_gpu_graded.py).
 """
 import numpy as np
 import pytest
@@ -36,10 +41,15 @@ def solve(prec, H, wr, tile=128):
 
 
 @pytest.mark.parametrize("kind,n", [("gauss", 500), ("gauss", 1100), ("prescribed", 700),
-                                    ("graded", 1300), ("graded-wide", 900)])
+                                    ("graded", 1300), ("graded-wide", 900),
+                                    ("graded-rows", 1500), ("graded-400", 1200)])
 def test_fp32_against_fp64(kind, n):
     if kind == "gauss":
         H, shifts = case(n, seed=60 + n, m=30)
+    elif kind == "graded-rows":      # 10 % of the indices graded, k in [-400, 400]
+        H, shifts = case(n, seed=60 + n, m=40, kmax=400, frac=0.1)
+    elif kind == "graded-400":       # every index, k in [-400, 400]
+        H, shifts = case(n, seed=60 + n, m=40, kmax=400)
     else:
         H, lam = w.serving_problem(n, seed=40 + n, graded=False)
         if kind == "graded":
@@ -58,9 +68,11 @@ def test_fp32_against_fp64(kind, n):
     print(f"{kind} n={n}: raw 1-|cos| {e_raw:.2e} resid {rr.max():.2e}; "
           f"checked {e_cor:.2e} resid {r32.max():.2e}; fp64 resid {r64.max():.2e}")
     assert r32.max() <= 10 * n
-    assert e_raw <= 1e-10
+    assert e_raw <= 1e-10 and e_cor <= 1e-10
+    assert np.all(r32 <= np.maximum(rr, 10 * n))
     if kind.startswith("graded"):
-        assert np.array_equal(x32, x64)
-    else:
-        assert e_cor <= 1e-10
-        assert np.all(r32 <= np.maximum(rr, 10 * n))
+        from tests.test_gpu_graded import componentwise
+        rel = componentwise(x32, x64)
+        print(f"  componentwise vs FP64 above 1e-250: p99.9 {np.quantile(rel, 0.999):.2e} "
+              f"max {rel.max():.2e}")
+        assert np.quantile(rel, 0.999) <= 2e-3 and rel.max() <= 0.1

@@ -67,6 +67,21 @@ int choose_pack(int requested, int b, int total_shifts);
 // sweep workgroup for one shift per wavefront; 0 (no lookahead) for four.
 int lookahead_blocks_per_cu(int pack, int b);
 
+// Planning (hsrq_sched.cpp).  A contiguous range of one problem's shifts.
+// Shifts are independent given H, so a problem can be cut into pieces that
+// run in different lanes; every shift's arithmetic is the same in any piece
+// (results are bit-identical).
+struct Piece {
+  int prob, k0, m;
+};
+// Tile of the automatic policy for a problem of order n among `count`.
+int auto_tile(int n, int count);
+// Lanes of a plan: groups of pieces and the tile of each group.
+void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz,
+                  std::vector<std::vector<Piece>>& groups, std::vector<int>& tiles);
+// Start step of each problem of a lane of `steps` steps.
+std::vector<int> step_offsets(const std::vector<ProbDev>& probs, int steps);
+
 // Device workspace of one problem (bytes, 256-aligned pieces).
 size_t prob_workspace_bytes(int n, int m, int b);
 // Fills the descriptor of problem q with tile b, start vector `its` (or per

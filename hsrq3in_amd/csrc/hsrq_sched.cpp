@@ -135,4 +135,118 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.status = (int*)(ws + L.st);
 }
 
+// Tile of the automatic policy.  A single problem below n = 4096 takes
+// 64-row tiles (half the step latency of 128); in a batch 128-row tiles win:
+// 30.3 ms vs 33.2-34.4 ms for 64-row tiles on the small problems and 44-57
+// ms with 256-row tiles (MI355X, 256 problems of n in [200, 4000]).
+int auto_tile(int n, int count) { return count == 1 && n < 4096 ? 64 : 128; }
+
+
+// Lane assignment.  Problems are grouped by tile.  Few problems (< 8): each
+// problem's shifts are cut into `nl` pieces, one per lane, so the diagonal
+// sweeps of one lane overlap the panel passes of another.  Many problems: a
+// group with many shifts is split into up to 3 lanes of whole problems,
+// balanced by panel-pass work m n^2, largest first (256-problem batch: 1
+// lane 34.7 ms, 2 lanes 30.8, 3 lanes 30.3, 4 lanes 34.3).
+void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz,
+                  std::vector<std::vector<Piece>>& groups, std::vector<int>& tiles) {
+  const bool fixed = tlsz > 0 || ctx->tile > 0;
+  std::vector<int> tile_of(count);
+  int nmax = 0;
+  for (int i = 0; i < count; ++i) nmax = std::max(nmax, in[i].n);
+  for (int i = 0; i < count; ++i)
+    tile_of[i] = fixed ? choose_tile(ctx, tlsz, nmax)
+                       : choose_tile(ctx, auto_tile(in[i].n, count), 0);
+  std::vector<int> distinct(tile_of);
+  std::sort(distinct.begin(), distinct.end());
+  distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+  for (int b : distinct) {
+    std::vector<int> g;
+    long long shifts = 0;
+    for (int i = 0; i < count; ++i)
+      if (tile_of[i] == b) {
+        g.push_back(i);
+        shifts += in[i].m;
+      }
+    if (count < 8) {
+      int nl = ctx->lanes > 0 ? ctx->lanes : (shifts >= kSplitMinShifts ? 2 : 1);
+      std::vector<std::vector<Piece>> h(nl);
+      for (int i : g) {
+        // cut in multiples of 16 shifts (whole packed workgroups)
+        const int m = in[i].m, per = ((m + nl - 1) / nl + 15) / 16 * 16;
+        for (int l = 0; l < nl; ++l) {
+          const int k0 = std::min(m, l * per), k1 = std::min(m, k0 + per);
+          if (k1 > k0 || (l == 0 && m == 0)) h[l].push_back({i, k0, k1 - k0});
+        }
+      }
+      for (auto& x : h)
+        if (!x.empty()) {
+          groups.push_back(x);
+          tiles.push_back(b);
+        }
+      continue;
+    }
+    const int nl = ctx->lanes > 0
+                       ? ctx->lanes
+                       : (int)std::min<long long>(3, std::max<long long>(1, shifts / (2 * kPackedMinShifts)));
+    std::vector<int> ord(g);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) {
+      return (double)in[a].m * in[a].n * in[a].n > (double)in[c].m * in[c].n * in[c].n;
+    });
+    std::vector<std::vector<Piece>> h(nl);
+    std::vector<double> load(nl, 0.0);
+    for (int i : ord) {
+      const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      h[t].push_back({i, 0, in[i].m});
+      load[t] += (double)in[i].m * in[i].n * in[i].n;
+    }
+    for (auto& x : h) {
+      std::sort(x.begin(), x.end(), [](const Piece& u, const Piece& v) { return u.prob < v.prob; });
+      if (!x.empty()) {
+        groups.push_back(x);
+        tiles.push_back(b);
+      }
+    }
+  }
+}
+
+// Start steps of a lane's problems: largest T m first, each where it raises
+// the per-step maximum of the shifts in flight least (then their sum of
+// squares).  Weighting the steps by their panel-pass work as well (rows above
+// the tile) measured no better on the 256-problem batch.
+std::vector<int> step_offsets(const std::vector<ProbDev>& probs, int steps) {
+  const int cnt = (int)probs.size();
+  std::vector<int> off(cnt, 0);
+  if (cnt < 2) return off;
+  std::vector<int> order;
+  for (int i = 0; i < cnt; ++i)
+    if (probs[i].m > 0) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (int64_t)probs[a].T * probs[a].m > (int64_t)probs[b].T * probs[b].m;
+  });
+  std::vector<double> load(steps, 0.0);
+  for (int i : order) {
+    const int T = probs[i].T;
+    const double w = probs[i].m;
+    int best = 0;
+    double bmax = 0.0, bsq = 0.0;
+    for (int o = 0; o + T <= steps; ++o) {
+      double mx = 0.0, sq = 0.0;
+      for (int s = o; s < o + T; ++s) {
+        const double x = load[s] + w;
+        mx = std::max(mx, x);
+        sq += x * x;
+      }
+      if (o == 0 || mx < bmax || (mx == bmax && sq < bsq)) {
+        best = o;
+        bmax = mx;
+        bsq = sq;
+      }
+    }
+    off[i] = best;
+    for (int s = best; s < best + T; ++s) load[s] += w;
+  }
+  return off;
+}
+
 }  // namespace hsrq

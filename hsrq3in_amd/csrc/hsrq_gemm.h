@@ -16,7 +16,8 @@ constexpr int kGemmColsRes = 64;      //   and validator
 
 // One product.  Mode 0: C = A B (A n x n, B n x m).  Mode 1 (validator):
 // A = H (upper Hessenberg, hess = 1), B = X, lam = shifts; part (3 x panels
-// x m Blue accumulators), rsmax (max row 1-norm per 64-row block) and res (m
+// x m Blue accumulators), rsmax (max row 1-norm per 128-row panel, from the
+// panel's first column block) and res (m
 // residuals ||(H - lam_k I) x_k||_2 / (||H||_oo ||x_k||_2 2^-52)) in the
 // workspace of resid_describe.
 struct GemmDesc {
@@ -34,14 +35,12 @@ struct GemmDesc {
 };
 
 struct GemmTileItem {   // product block (prob, row panel, column block);
-  int prob, panel, cb, pad;   // row max: (prob, row block); final: cb = column
+  int prob, panel, cb, pad;   // final pass: cb = column
 };
 
-// Items of one product (XCD-aware order; bn: kGemmColsBT or kGemmColsRes),
-// of the row maxima and of the
-// per-column final pass of problem `prob`.
+// Items of one product (XCD-aware order; bn: kGemmColsBT or kGemmColsRes) and
+// of the per-column final pass of problem `prob`.
 void add_gemm_items(int prob, int n, int m, int bn, std::vector<GemmTileItem>& out);
-void add_rowmax_items(int prob, int n, std::vector<GemmTileItem>& out);
 void add_resid_final_items(int prob, int m, std::vector<GemmTileItem>& out);
 
 // Validator workspace of one problem (doubles) and its descriptor.
@@ -51,9 +50,9 @@ void resid_describe(GemmDesc& D, int n, const double* H, int64_t ldH, int m, con
 
 hipError_t launch_gemm(const GemmDesc* d_descs, const GemmTileItem* items, int count,
                        hipStream_t st);
-// Row maxima, the product with its reduction, the final pass: three launches.
-hipError_t launch_resid(const GemmDesc* d_descs, const GemmTileItem* rowmax, int n_rowmax,
-                        const GemmTileItem* tiles, int n_tiles, const GemmTileItem* fin,
-                        int n_fin, hipStream_t st);
+// The product with its reductions (residual sums and row 1-norms), then the
+// final pass: two launches.
+hipError_t launch_resid(const GemmDesc* d_descs, const GemmTileItem* tiles, int n_tiles,
+                        const GemmTileItem* fin, int n_fin, hipStream_t st);
 
 }  // namespace hsrq

@@ -52,6 +52,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
   __shared__ double As[2][BK * LDA];
   __shared__ double Bs[2][BN * LDB];
   __shared__ double red[3][2][MODE == 1 ? BN : 1];   // mode 1: the two row halves
+  __shared__ double rsh[MODE == 1 ? 256 : 1];         // mode 1: partial row 1-norms
 
   const GemmTileItem it = items[blockIdx.x];
   const GemmDesc& D = descs[it.prob];
@@ -67,6 +68,9 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
 
   constexpr int NPA = BK * BM / 256, NPB = BK * BN / 256;
   double pa[NPA], pb[NPB];
+  // mode 1: ||H(i,:)||_1 of the thread's row (tid % 128: the same for every
+  // element the thread fetches), over its half of the columns, in fetch order
+  double rsum = 0.0;
   auto fetch = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < NPA; ++i) {
@@ -76,6 +80,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
       // (Hessenberg: nothing left of the subdiagonal is read)
       const bool in = rr < nr && k < n && (!D.hess || k >= r0 + rr - 1);
       pa[i] = in ? A[(r0 + rr) + (int64_t)k * ldA] : 0.0;
+      if (MODE == 1) rsum += fabs(pa[i]);
     }
 #pragma unroll
     for (int i = 0; i < NPB; ++i) {
@@ -182,35 +187,24 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc* __restrict__ descs
         red[2][wm][col] = s.abig;
       }
     }
+  rsh[tid] = rsum;
   __syncthreads();
   if (tid < nc) {
     double* __restrict__ part = D.part + 3 * ((int64_t)it.panel * m + c0 + tid);
 #pragma unroll
     for (int q = 0; q < 3; ++q) part[q] = red[q][0][tid] + red[q][1][tid];
   }
-}
-
-// ||H(i,:)||_1 for the 64 rows of a row block (Hessenberg part), then the
-// block's maximum: rsmax[rb].  Four column groups per workgroup (waves), each
-// summing every fourth column, combined in a fixed order.
-__global__ __launch_bounds__(256) void k_rowmax(const GemmDesc* __restrict__ descs,
-                                                const GemmTileItem* __restrict__ items) {
-  __shared__ double sh[4][64];
-  const GemmTileItem it = items[blockIdx.x];
-  const GemmDesc& D = descs[it.prob];
-  const int n = D.n;
-  const int i = it.panel * 64 + (threadIdx.x & 63);
-  const int g = threadIdx.x >> 6;
-  double a = 0.0;
-  if (i < n)
-    for (int j = max(0, i - 1) + g; j < n; j += 4) a += fabs(D.A[i + (int64_t)j * D.ldA]);
-  sh[g][threadIdx.x & 63] = a;
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    const double s = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
-                     (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
-    const double mx = wave_max(s);
-    if (threadIdx.x == 0) D.rsmax[it.panel] = mx;
+  // the panel's largest row 1-norm (the first column block of the panel
+  // writes it; every block reads the same rows of H)
+  if (it.cb == 0 && tid < 64) {
+    double mx = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int row = tid + 64 * h;
+      if (row < nr) mx = fmax(mx, rsh[row] + rsh[row + 128]);
+    }
+    mx = wave_max(mx);
+    if (tid == 0) D.rsmax[it.panel] = mx;
   }
 }
 
@@ -249,8 +243,7 @@ __global__ __launch_bounds__(256) void k_resid_final(const GemmDesc* __restrict_
       rs.abig += q[2];
     }
     double hmax = 0.0;
-    const int nrb = (n + 63) / 64;
-    for (int rb = 0; rb < nrb; ++rb) hmax = fmax(hmax, D.rsmax[rb]);
+    for (int p = 0; p < np; ++p) hmax = fmax(hmax, D.rsmax[p]);
     D.res[k] = acc3_norm(rs) / (hmax * acc3_norm(xs) * 0x1p-52);
   }
 }
@@ -270,17 +263,13 @@ void add_gemm_items(int prob, int n, int m, int bn, std::vector<GemmTileItem>& o
   }
 }
 
-void add_rowmax_items(int prob, int n, std::vector<GemmTileItem>& out) {
-  for (int rb = 0; rb < (n + 63) / 64; ++rb) out.push_back(GemmTileItem{prob, rb, 0, 0});
-}
-
 void add_resid_final_items(int prob, int m, std::vector<GemmTileItem>& out) {
   for (int k = 0; k < m; ++k) out.push_back(GemmTileItem{prob, 0, k, 0});
 }
 
 size_t resid_workspace_doubles(int n, int m) {
-  const size_t np = (n + BM - 1) / BM, nrb = (n + 63) / 64;
-  return 3 * np * (size_t)m + nrb + (size_t)m;
+  const size_t np = (n + BM - 1) / BM;
+  return 3 * np * (size_t)m + np + (size_t)m;
 }
 
 void resid_describe(GemmDesc& D, int n, const double* H, int64_t ldH, int m, const double* wr,
@@ -294,10 +283,10 @@ void resid_describe(GemmDesc& D, int n, const double* H, int64_t ldH, int m, con
   D.n = n;
   D.m = m;
   D.hess = 1;
-  const size_t np = (n + BM - 1) / BM, nrb = (n + 63) / 64;
+  const size_t np = (n + BM - 1) / BM;
   D.part = ws;
   D.rsmax = ws + 3 * np * (size_t)m;
-  D.res = D.rsmax + nrb;
+  D.res = D.rsmax + np;
 }
 
 hipError_t launch_gemm(const GemmDesc* d_descs, const GemmTileItem* items, int count,
@@ -307,11 +296,9 @@ hipError_t launch_gemm(const GemmDesc* d_descs, const GemmTileItem* items, int c
   return hipGetLastError();
 }
 
-hipError_t launch_resid(const GemmDesc* d_descs, const GemmTileItem* rowmax, int n_rowmax,
-                        const GemmTileItem* tiles, int n_tiles, const GemmTileItem* fin,
-                        int n_fin, hipStream_t st) {
+hipError_t launch_resid(const GemmDesc* d_descs, const GemmTileItem* tiles, int n_tiles,
+                        const GemmTileItem* fin, int n_fin, hipStream_t st) {
   if (n_fin <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rowmax, dim3(n_rowmax), dim3(256), 0, st, d_descs, rowmax);
   hipLaunchKernelGGL((k_gemm<1, 2>), dim3(n_tiles), dim3(256), 0, st, d_descs, tiles);
   hipLaunchKernelGGL(k_resid_final, dim3(n_fin), dim3(256), 0, st, d_descs, fin);
   return hipGetLastError();

@@ -63,23 +63,21 @@ int residuals_run(hsrq_context* ctx, int count, const hsrq_problem* probs, doubl
   double* ws = nullptr;
   HIP_TRY(hipMalloc(&ws, std::max<size_t>(off[count], 1) * sizeof(double)));
   std::vector<GemmDesc> descs(count);
-  std::vector<GemmTileItem> rowmax, tiles, fin;
+  std::vector<GemmTileItem> tiles, fin;
   for (int i = 0; i < count; ++i) {
     const hsrq_problem& q = probs[i];
     resid_describe(descs[i], q.n, q.H, q.ldH, q.m, q.wr, q.vr, q.ldvr, ws + off[i]);
     if (q.m == 0) continue;
-    add_rowmax_items(i, q.n, rowmax);
     add_gemm_items(i, q.n, q.m, kGemmColsRes, tiles);
     add_resid_final_items(i, q.m, fin);
   }
   std::vector<char> h;
-  const size_t od = put(h, descs), o1 = put(h, rowmax), o2 = put(h, tiles), o3 = put(h, fin);
+  const size_t od = put(h, descs), o2 = put(h, tiles), o3 = put(h, fin);
   char* d = nullptr;
   int rc = upload(ctx, h, &d);
   hipError_t e = hipSuccess;
   if (!rc)
-    e = launch_resid((const GemmDesc*)(d + od), (const GemmTileItem*)(d + o1), (int)rowmax.size(),
-                     (const GemmTileItem*)(d + o2), (int)tiles.size(),
+    e = launch_resid((const GemmDesc*)(d + od), (const GemmTileItem*)(d + o2), (int)tiles.size(),
                      (const GemmTileItem*)(d + o3), (int)fin.size(), ctx->stream);
   size_t pos = 0;
   for (int i = 0; i < count && !rc && e == hipSuccess; ++i) {

@@ -325,7 +325,7 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   std::vector<GemmItem> part;
   std::vector<RowItem> fin, init;
   std::vector<ShiftItem> bt;
-  std::vector<GemmTileItem> vrow, vtile, vfin;   // validator of the finishing requests
+  std::vector<GemmTileItem> vtile, vfin;   // validator of the finishing requests
   std::vector<GemmDesc> gdescs;
   std::map<int, StepItems> steps;   // by tile size
   std::vector<Req*> fresh, fin_reqs;
@@ -366,7 +366,6 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
                      r->dvr + (size_t)r->first * r->ldvr, r->ldvr,
                      (double*)(r->ws + prob_workspace_bytes(r->in.n, r->in.m, r->b)));
       gdescs.push_back(G);
-      add_rowmax_items(r->slot, r->in.n, vrow);
       add_gemm_items(r->slot, r->in.n, mm, kGemmColsRes, vtile);
       add_resid_final_items(r->slot, mm, vfin);
     }
@@ -377,7 +376,6 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   // pack the item tables into the pinned staging buffer of this parity
   size_t need = align_up(part.size() * sizeof(GemmItem)) + align_up(fin.size() * sizeof(RowItem)) +
                 align_up(init.size() * sizeof(RowItem)) + align_up(bt.size() * sizeof(ShiftItem)) +
-                align_up(vrow.size() * sizeof(GemmTileItem)) +
                 align_up(vtile.size() * sizeof(GemmTileItem)) +
                 align_up(vfin.size() * sizeof(GemmTileItem));
   for (auto& kv : steps)
@@ -413,8 +411,6 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   const RowItem* d_fin = (const RowItem*)put(fin.data(), fin.size() * sizeof(RowItem));
   const RowItem* d_init = (const RowItem*)put(init.data(), init.size() * sizeof(RowItem));
   const ShiftItem* d_bt = (const ShiftItem*)put(bt.data(), bt.size() * sizeof(ShiftItem));
-  const GemmTileItem* d_vrow =
-      (const GemmTileItem*)put(vrow.data(), vrow.size() * sizeof(GemmTileItem));
   const GemmTileItem* d_vtile =
       (const GemmTileItem*)put(vtile.data(), vtile.size() * sizeof(GemmTileItem));
   const GemmTileItem* d_vfin =
@@ -471,8 +467,8 @@ int enqueue_tick(hsrq_server* s, int par, int& rc) {
   for (const Dev& x : dev)
     if ((e = launch_update(P, x.gb, x.ngb, s->stream)) != hipSuccess) return fail(e);
   if ((e = launch_backtransform(P, d_bt, (int)bt.size(), s->stream)) != hipSuccess) return fail(e);
-  if ((e = launch_resid(s->d_gdesc, d_vrow, (int)vrow.size(), d_vtile, (int)vtile.size(), d_vfin,
-                        (int)vfin.size(), s->stream)) != hipSuccess)
+  if ((e = launch_resid(s->d_gdesc, d_vtile, (int)vtile.size(), d_vfin, (int)vfin.size(),
+                        s->stream)) != hipSuccess)
     return fail(e);
   for (size_t i = 0; i < fin_reqs.size(); ++i) {
     Req* r = fin_reqs[i];

@@ -86,11 +86,24 @@ __global__ __launch_bounds__(256) void k_norm_final(const ProbDev* __restrict__ 
   if (r < nb) {
     const int rb = tile_first_rowblock(P, I) + r / kRowBlock;
     const int cbd0 = tile_first_rowblock(P, I), cbd1 = cbd0 + tile_num_rowblocks(P, I);
-    for (int cb = 0; cb < P.nrb; ++cb) {
-      if (!norm_block_needed(P, rb, cb)) continue;
-      const double v = P.part[(int64_t)cb * P.n + l + r];
-      rs += v;
-      if (cb >= cbd0 && cb < cbd1) ds += v;
+    // the column blocks a row needs are cb >= cb_lo (the Hessenberg part):
+    // loaded 8 at a time (independent loads in flight), added in cb order
+    int cb_lo = 0;
+    while (cb_lo < P.nrb && !norm_block_needed(P, rb, cb_lo)) ++cb_lo;
+    const double* __restrict__ pc = P.part + l + r;
+    for (int cb0 = cb_lo; cb0 < P.nrb; cb0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        v[q] = cb0 + q < P.nrb ? pc[(int64_t)(cb0 + q) * P.n] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int cb = cb0 + q;
+        if (cb < P.nrb) {
+          rs += v[q];
+          if (cb >= cbd0 && cb < cbd1) ds += v[q];
+        }
+      }
     }
   }
   const int wave = threadIdx.x >> 6;

@@ -488,6 +488,8 @@ def main():
     ap.add_argument("--batch-tile", type=int, default=0)
     ap.add_argument("--packing", type=int, default=0)
     ap.add_argument("--batch-packing", type=int, default=0)
+    ap.add_argument("--batch-lanes", type=int, default=0,
+                    help="lanes of the batch plan (with --batch-tile: no autotuning)")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -603,6 +605,8 @@ def main():
         out["around_the_path"] = around_path(ctx)
     if not args.no_batch:
         ctx.set_packing(args.batch_packing)
+        if args.batch_lanes:
+            ctx.set_lanes(args.batch_lanes)
         out["batched"] = batched(ctx, max(1, args.steps // 2), 1, args.batch_tile, 256)
         out["batched"]["packing"] = args.batch_packing
     if args.precision_study:

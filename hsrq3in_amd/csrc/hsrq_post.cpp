@@ -57,16 +57,25 @@ size_t put(std::vector<char>& h, const std::vector<T>& v) {
 }  // namespace
 
 int residuals_run(hsrq_context* ctx, int count, const hsrq_problem* probs, double* res) {
+  // per problem its workspace; the residuals of all problems contiguous at
+  // the end (one copy to the host)
   std::vector<size_t> off(count + 1, 0);
-  for (int i = 0; i < count; ++i)
+  size_t total_m = 0;
+  for (int i = 0; i < count; ++i) {
     off[i + 1] = off[i] + align_up(resid_workspace_doubles(probs[i].n, probs[i].m) * 8) / 8;
+    total_m += probs[i].m;
+  }
   double* ws = nullptr;
-  HIP_TRY(hipMalloc(&ws, std::max<size_t>(off[count], 1) * sizeof(double)));
+  HIP_TRY(hipMalloc(&ws, std::max<size_t>(off[count] + total_m, 1) * sizeof(double)));
+  double* const res_all = ws + off[count];
   std::vector<GemmDesc> descs(count);
   std::vector<GemmTileItem> tiles, fin;
+  size_t rpos = 0;
   for (int i = 0; i < count; ++i) {
     const hsrq_problem& q = probs[i];
     resid_describe(descs[i], q.n, q.H, q.ldH, q.m, q.wr, q.vr, q.ldvr, ws + off[i]);
+    descs[i].res = res_all + rpos;
+    rpos += q.m;
     if (q.m == 0) continue;
     add_gemm_items(i, q.n, q.m, kGemmColsRes, tiles);
     add_resid_final_items(i, q.m, fin);
@@ -79,13 +88,9 @@ int residuals_run(hsrq_context* ctx, int count, const hsrq_problem* probs, doubl
   if (!rc)
     e = launch_resid((const GemmDesc*)(d + od), (const GemmTileItem*)(d + o2), (int)tiles.size(),
                      (const GemmTileItem*)(d + o3), (int)fin.size(), ctx->stream);
-  size_t pos = 0;
-  for (int i = 0; i < count && !rc && e == hipSuccess; ++i) {
-    if (probs[i].m > 0)
-      e = hipMemcpyAsync(res + pos, descs[i].res, probs[i].m * sizeof(double),
-                         hipMemcpyDeviceToHost, ctx->stream);
-    pos += probs[i].m;
-  }
+  if (!rc && e == hipSuccess && total_m > 0)
+    e = hipMemcpyAsync(res, res_all, total_m * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx->stream);
   if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(ws);
   if (rc) return rc;

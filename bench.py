@@ -19,6 +19,7 @@ algorithm) timed on the host's cores on the same workload.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -145,6 +146,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     for _ in range(warmup):
         plan.execute(graph=True)
     plan.sync()
+    gc.collect()
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.execute(graph=True)
@@ -152,6 +154,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     ms = 1e3 * (time.perf_counter() - t0) / steps
     st = plan.status()
     good, pos, worst = 0, 0, 0.0
+    gc.collect()
     t0 = time.perf_counter()
     allres = ctx.residuals_batch(items)   # every vector, one device pass
     val_ms = 1e3 * (time.perf_counter() - t0)
@@ -172,6 +175,7 @@ def batched(ctx, steps, warmup, tile, plain_sample):
     for Ht, wrt, vrt in sample[:2]:
         ctx.solve(Ht, wrt, vrt, 0)
     torch.cuda.synchronize()
+    gc.collect()
     t0 = time.perf_counter()
     sm = 0
     for Ht, wrt, vrt in sample:
@@ -475,6 +479,11 @@ def served(ctx, qps_list, nreq, pool_size, seed=9):
 
 
 def main():
+    # No automatic garbage collection during the run: once the batch's
+    # objects exist a collection pause takes tens of milliseconds and would
+    # land in whichever timed region it hits.  The timed regions collect
+    # before they start.
+    gc.disable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -537,6 +546,7 @@ def main():
         plan.execute(graph=graph)
     plan.sync()
     torch.cuda.synchronize()
+    gc.collect()
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):

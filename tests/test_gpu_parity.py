@@ -125,17 +125,25 @@ def test_exact_norms_identical(ctx):
     assert np.array_equal(x1, x2)
 
 
-def test_packings_agree(ctx):
+@pytest.mark.parametrize("n,tile,kmax", [(700, 128, 200), (700, 128, 0), (517, 64, 60),
+                                          (901, 256, 400), (1290, 128, 400), (250, 256, 30)])
+def test_packings_agree(ctx, n, tile, kmax):
     """1 and 4 shifts per wavefront run the same arithmetic (no implicit FMA
-    contraction, explicit fma()), so the vectors are bit-identical."""
+    contraction, explicit fma()), so the vectors are bit-identical.  The
+    four-shift sweep defers the rows above each 16-row block and falls back to
+    full steps where a shift needs a protected step: graded inputs (kmax > 0)
+    exercise the fall-backs at many places of the blocks, ragged first tiles
+    the partial blocks."""
     from hsrq3in_amd import Context
-    H, wr = case(700, seed=21, m=20, kmax=200)
+    H, wr = case(n, seed=21 + n, m=20, kmax=kmax)
     out = []
     for pk in (1, 4):
         c = Context(0, packing=pk)
-        out.append(gpu_solve(c, H, wr, 128)[0])
+        x, _, st, _ = gpu_solve(c, H, wr, tile)
+        out.append((x, st))
         c.close()
-    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
 
 
 @pytest.mark.parametrize("lanes", [2, 3])

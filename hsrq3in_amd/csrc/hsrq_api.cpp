@@ -200,7 +200,8 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single, D
     HIP_TRY(hipMemcpy(L->d_probs, L->probs.data(), cnt * sizeof(ProbDev), hipMemcpyHostToDevice));
   if (L->own_stream) HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
   const int la = !single             ? 0
-                 : ctx->lookahead < 0 ? lookahead_blocks_per_cu(L->pack, L->b)
+                 : ctx->lookahead < 0 ? lookahead_blocks_per_cu(L->pack, L->b, total_lane_shifts,
+                                                                ctx->cus)
                                       : ctx->lookahead;
   if (la > 0) {   // 0: off (asked for, or the automatic choice for packed sweeps)
     HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
@@ -251,7 +252,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       p->pieces.push_back({(int)l, (int)j, pc.prob, pc.k0});
       p->tile_of[pc.prob] = tiles[l];
       ws_off[l].push_back(bytes);
-      bytes += prob_workspace_bytes(in[pc.prob].n, pc.m, tiles[l]);
+      bytes += prob_workspace_bytes(in[pc.prob].n, pc.m, tiles[l],
+                                    ctx->precision != HSRQ_PREC_FP64);
     }
   p->work_bytes = bytes;
   hipError_t e = hipSuccess;
@@ -288,7 +290,8 @@ int plan_build(hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz, i
       q.vr = q.vr + (size_t)pc.k0 * q.ldvr;
       ProbDev P;
       const int* itk = its_k_of && its_k_of[pc.prob] ? its_k_of[pc.prob] + pc.k0 : nullptr;
-      prob_describe(P, q, L->b, its, ctx->exact, (char*)p->d_work + ws_off[l][j], itk);
+      prob_describe(P, q, L->b, its, ctx->exact, (char*)p->d_work + ws_off[l][j], itk,
+                    L->prec != HSRQ_PREC_FP64);
       L->probs.push_back(P);
       shifts += pc.m;
     }
@@ -347,6 +350,10 @@ int lane_enqueue(Lane* L, Prof* prof) {
   if (rc) return rc;
   rc = run(1, st, [&](hipStream_t s) { return launch_init(L->d_probs, L->init, L->n_init, s); });
   if (rc) return rc;
+  if (L->prec != HSRQ_PREC_FP64)   // equilibration exponents of the FP32 panel pass
+    for (size_t i = 0; i < L->probs.size(); ++i)
+      if (L->probs[i].m > 0)
+        HIP_TRY(launch_equilibrate(L->d_probs + i, L->probs[i].n, L->probs[i].nrb, st));
   if (L->side) {
     HIP_TRY(hipEventRecord(L->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(sd, L->ev_fork, 0));

@@ -64,8 +64,9 @@ int choose_tile(const hsrq_context* ctx, int tlsz, int n);
 int choose_pack(int requested, int b, int total_shifts);
 // Workgroups per CU of a panel pass that runs beside the diagonal sweeps of
 // this layout and tile, automatic choice: the most that leave room for a
-// sweep workgroup for one shift per wavefront; 0 (no lookahead) for four.
-int lookahead_blocks_per_cu(int pack, int b);
+// sweep workgroup for one shift per wavefront, when the lane's shifts take at
+// most one sweep workgroup per CU (cus CUs); 0 (no lookahead) otherwise.
+int lookahead_blocks_per_cu(int pack, int b, int shifts, int cus);
 
 // Planning (hsrq_sched.cpp).  A contiguous range of one problem's shifts.
 // Shifts are independent given H, so a problem can be cut into pieces that
@@ -83,12 +84,14 @@ void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, in
 std::vector<int> step_offsets(const std::vector<ProbDev>& probs, int steps);
 
 // Device workspace of one problem (bytes, 256-aligned pieces).
-size_t prob_workspace_bytes(int n, int m, int b);
+// Workspace bytes of one problem; eq: with the FP32 panel pass's operands.
+size_t prob_workspace_bytes(int n, int m, int b, bool eq = false);
 // Fills the descriptor of problem q with tile b, start vector `its` (or per
 // shift its_k, device, m entries) and the workspace at `ws`
-// (prob_workspace_bytes(q.n, q.m, b) bytes).
+// (prob_workspace_bytes(q.n, q.m, b) bytes).  eq: the problem runs the FP32
+// panel pass (its equilibration exponents are set up).
 void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws,
-                   const int* its_k = nullptr);
+                   const int* its_k = nullptr, bool eq = false);
 
 // Item tables.  `i` is the problem's index in the descriptor array.  The
 // builders append to any container with push_back (std::vector, or the
@@ -151,6 +154,9 @@ inline void add_step_items(const ProbDev& P, int i, int J, int pack, StepItems& 
 
 // Launchers (hsrq_kernels.hip).  Zero counts launch nothing.
 hipError_t launch_norms(const ProbDev*, const GemmItem*, int, const RowItem*, int, hipStream_t);
+// Equilibration exponents of H for the FP32 panel pass (ProbDev::ccol, rrow)
+// of problem d_prob (device descriptor) of order n with nrb row blocks.
+hipError_t launch_equilibrate(const ProbDev* d_prob, int n, int nrb, hipStream_t);
 hipError_t launch_init(const ProbDev*, const RowItem*, int, hipStream_t);
 hipError_t launch_diag(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);
 hipError_t launch_diag16(int rpl, const ProbDev*, const K1Item*, int, hipStream_t);

@@ -48,6 +48,18 @@ struct ProbDev {
   const int* its_k;   // if set: the start vector of shift k is its_k[k] (-1 or
                       // >= 0 as its), overriding its
   int force_exact;    // 1: always use exact norms in the diagonal sweep
+  // FP32 panel pass only (null otherwise): power-of-two equilibration of H,
+  // ccol[j] = -ilogb(max_i |H(i, j)|), rrow[i] = ilogb(max_j |H(i, j)| 2^ccol[j])
+  // (n each), and per tile-column parity the exponents of the [W | Z]
+  // columns scaled by 2^-ccol of their panel columns, Be[s & 1][c] (2m each)
+  int* ccol;
+  int* rrow;
+  int* Be[2];
+  // and the operands in float: H(i, j) 2^(ccol[j] - rrow[i]) over the
+  // Hessenberg part (ld n), the [W | Z] columns scaled by 2^(-ccol - Be) per
+  // parity (b x 2m, ld b)
+  float* H32;
+  float* B32[2];
 };
 
 struct K1Item {

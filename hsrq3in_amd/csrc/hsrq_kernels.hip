@@ -169,6 +169,79 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
 }
 
 // ---------------------------------------------------------------------------
+// Power-of-two equilibration of H for the FP32 panel pass (once per solve):
+// ccol[j] = -ilogb(max_i |H(i, j)|) over the Hessenberg part of column j, then
+// rrow[i] = ilogb(max_j |H(i, j)| 2^ccol[j]).  For H = D H0 D^-1 with D a
+// diagonal of powers of two the scaled entries H(i, j) 2^(ccol[j] - rrow[i])
+// are of the size of H0's however wide D's range, so the conversion to float
+// keeps them.  One wavefront per column; rows in blocks of 64 (threads read a
+// column's consecutive rows together).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_eq_col(const ProbDev* __restrict__ probs) {
+  const ProbDev& P = probs[0];
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= P.n) return;   // whole wavefront
+  const int rows = min(j + 2, P.n);
+  const double* __restrict__ col = P.H + (int64_t)j * P.ldH;
+  double a = 0.0;
+  for (int i = lane; i < rows; i += 64) a = fmax(a, fabs(col[i]));
+  a = wave_max(a);
+  if (lane == 0) P.ccol[j] = (a > 0.0 && isfinite(a)) ? -ilogb(a) : 0;
+}
+
+// Row maxima in two passes: block (64 rows, column chunk blockIdx.y of
+// gridDim.y) into part[chunk * n + row], then the maximum over the chunks.
+__global__ __launch_bounds__(256) void k_eq_row(const ProbDev* __restrict__ probs) {
+  const ProbDev& P = probs[0];
+  __shared__ double red[4][64];
+  const int r = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + r;
+  const int64_t ldH = P.ldH;
+  const int j0 = max((int)blockIdx.x * 64 - 1, 0);   // signed: block 0 starts at column 0
+  const int len = (P.n - j0 + gridDim.y - 1) / gridDim.y;
+  const int ja = j0 + blockIdx.y * len, jb = min(ja + len, P.n);
+  double a = 0.0;
+  for (int j = ja + ph; j < jb; j += 4)
+    if (i < P.n && j >= i - 1) a = fmax(a, ldexp(fabs(P.H[i + (int64_t)j * ldH]), P.ccol[j]));
+  red[ph][r] = a;
+  __syncthreads();
+  if (ph == 0 && i < P.n)
+    P.part[(int64_t)blockIdx.y * P.n + i] =
+        fmax(fmax(red[0][r], red[1][r]), fmax(red[2][r], red[3][r]));
+}
+
+__global__ __launch_bounds__(256) void k_eq_rowfin(const ProbDev* __restrict__ probs, int chunks) {
+  const ProbDev& P = probs[0];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= P.n) return;
+  double a = 0.0;
+  for (int c = 0; c < chunks; ++c) a = fmax(a, P.part[(int64_t)c * P.n + i]);
+  P.rrow[i] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
+}
+
+// H32(i, j) = H(i, j) 2^(ccol[j] - rrow[i]) in float over the Hessenberg part
+// (rows i <= j + 1; the panel passes read no other), one wavefront per column.
+__global__ __launch_bounds__(256) void k_eq_conv(const ProbDev* __restrict__ probs) {
+  const ProbDev& P = probs[0];
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= P.n) return;
+  const int rows = min(j + 2, P.n);
+  const double* __restrict__ col = P.H + (int64_t)j * P.ldH;
+  float* __restrict__ out = P.H32 + (int64_t)j * P.n;
+  const int cj = P.ccol[j];
+  for (int i = lane; i < rows; i += 64) out[i] = (float)ldexp(col[i], cj - P.rrow[i]);
+}
+
+// Exponent of the largest |x_r| 2^-ccol[l - 1 + r] over a shift's coefficient
+// column x (rows r < nb of the tile starting at row l), 0 if x is 0: the scale
+// of the [W | Z] column in the FP32 panel pass.  Per lane: its rows' best.
+__device__ __forceinline__ double coef_exp_part(const ProbDev& P, int l, double x, int r, int nb,
+                                                double cur) {
+  if (r < nb && x != 0.0) cur = fmax(cur, (double)(ilogb(x) - P.ccol[l - 1 + r]));
+  return cur;
+}
+
+// ---------------------------------------------------------------------------
 // Scale exponents of the panel pass of tile column J for tile row I < J and
 // shift k: consolidate the scales of y_I and Z_J (update-task.c:68-88), then
 // one protection for the whole fused update (update-task.c:103-248):
@@ -796,6 +869,33 @@ __global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
   int p = 0;
   if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
 
+  if (P.ccol) {   // FP32 panel pass: exponents of the scaled [W | Z] columns
+    double ew = -1e300, ez = -1e300;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      ew = coef_exp_part(P, l, w[q], lane + 64 * q, nb, ew);
+      ez = coef_exp_part(P, l, ldexp(z[q], -p), lane + 64 * q, nb, ez);
+    }
+    ew = wave_max(ew);
+    ez = wave_max(ez);
+    const int iw = ew > -1e300 ? (int)ew : 0, iz = ez > -1e300 ? (int)ez : 0;
+    if (valid) {
+      if (lane == 0) {
+        P.Be[J & 1][2 * k] = iw;
+        P.Be[J & 1][2 * k + 1] = iz;
+      }
+      float* Fw = P.B32[J & 1] + (int64_t)(2 * k) * P.b;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const int r = lane + 64 * q;
+        if (r < nb) {
+          const int cr = P.ccol[l - 1 + r];
+          Fw[r] = (float)ldexp(w[q], -cr - iw);
+          Fw[P.b + r] = (float)ldexp(ldexp(z[q], -p), -cr - iz);
+        }
+      }
+    }
+  }
   if (valid) {
     double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
     double* Bz = Bw + P.b;
@@ -1273,6 +1373,33 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   int p = 0;
   if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
 
+  if (P.ccol) {   // FP32 panel pass: exponents of the scaled [W | Z] columns
+    double ew = -1e300, ez = -1e300;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      ew = coef_exp_part(P, l, w[q], t + 16 * q, nb, ew);
+      ez = coef_exp_part(P, l, ldexp(z[q], -p), t + 16 * q, nb, ez);
+    }
+    ew = row16_max(ew);
+    ez = row16_max(ez);
+    const int iw = ew > -1e300 ? (int)ew : 0, iz = ez > -1e300 ? (int)ez : 0;
+    if (valid) {
+      if (t == 0) {
+        P.Be[J & 1][2 * k] = iw;
+        P.Be[J & 1][2 * k + 1] = iz;
+      }
+      float* Fw = P.B32[J & 1] + (int64_t)(2 * k) * P.b;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) {
+        const int r = t + 16 * q;
+        if (r < nb) {
+          const int cr = P.ccol[l - 1 + r];
+          Fw[r] = (float)ldexp(w[q], -cr - iw);
+          Fw[P.b + r] = (float)ldexp(ldexp(z[q], -p), -cr - iz);
+        }
+      }
+    }
+  }
   if (valid) {
     double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
     double* Bz = Bw + P.b;
@@ -1522,12 +1649,13 @@ __global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ prob
 // ---------------------------------------------------------------------------
 // Panel pass with FP32 MFMA (v_mfma_f32_16x16x4_f32) and FP32 accumulation:
 // the optional reduced-precision update path.  Same blocking and epilogue as
-// k_update.  The block's panel and [W | Z] columns are equilibrated by powers
-// of two before the conversion to float: H(i, j) 2^(c_j - r_i) (column
-// exponents over the block's rows, then row exponents) and B(j, s)
-// 2^(-c_j - e_s), so graded matrices (entries 2^+-800 and more) keep their
-// small terms; the FP32 result is scaled back by 2^(r_i + e_s) and finished
-// in FP64 (cross-over terms, shift correction, scaled update of y).  FP32 MFMA
+// k_update.  The panel and [W | Z] columns are equilibrated by powers of two
+// before the conversion to float: H(i, j) 2^(c_j - r_i) (column exponents of
+// H, then row exponents, once per solve: k_eq_col, k_eq_row) and B(j, s)
+// 2^(-c_j - e_s) (e_s from the sweep that emits the column), so graded
+// matrices (entries 2^+-800 and more) keep their small terms; the FP32 result
+// is scaled back by 2^(r_i + e_s) and finished in FP64 (cross-over terms,
+// shift correction, scaled update of y).  FP32 MFMA
 // layout: D[i][j] in register i%4 of lane j + 16 (i/4); here D = D^T[col][row],
 // so a lane holds one row and four adjacent columns, W (even) and Z (odd).
 // ---------------------------------------------------------------------------
@@ -1542,8 +1670,7 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
   // power-of-two equilibration of the block: column exponents of the panel
   // (b <= 256 columns), row exponents of the column-scaled panel, exponents
   // of the [W | Z] columns scaled by the inverse column exponents
-  __shared__ int ccol[256], rrow[GM], ecol[GN];
-  __shared__ double rmx[4][GM];
+  __shared__ int rrow[GM], ecol[GN];
 
   // LOOP: grid-stride over the items, so that a launch beside a diagonal
   // sweep can be capped to leave room for it on every CU; otherwise one
@@ -1567,10 +1694,12 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
     if (LOOP) asm volatile("" : "+v"(tid));
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t ldH = P.ldH;
 
-    const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;
     const double* __restrict__ Bg = P.Bm[J & 1];
+    // equilibrated float operands (k_eq_conv, the sweep): panel columns
+    // l-1 .. l+b-2, the block's rows; [W | Z] columns col0 ..
+    const float* __restrict__ Hf = P.H32 + (int64_t)(l - 1) * n + r0;
+    const float* __restrict__ Bf = P.B32[J & 1] + (int64_t)col0 * b;
 
     constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
     float pa[NPA], pb[NPB];
@@ -1579,17 +1708,13 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
       for (int i = 0; i < NPA; ++i) {
         const int idx = tid + 256 * i;
         const int kk = idx / GM, rr = idx - kk * GM;
-        pa[i] = (rr < nr) ? (float)ldexp(Hp[(r0 + rr) + (int64_t)(kc + kk) * ldH],
-                                         ccol[kc + kk] - rrow[rr])
-                          : 0.0f;
+        pa[i] = (rr < nr) ? Hf[rr + (int64_t)(kc + kk) * n] : 0.0f;
       }
 #pragma unroll
       for (int i = 0; i < NPB; ++i) {
         const int idx = tid + 256 * i;
         const int cc = idx / KC, k2 = idx - cc * KC;
-        pb[i] = (cc < ncols) ? (float)ldexp(Bg[(kc + k2) + (int64_t)(col0 + cc) * b],
-                                            -ccol[kc + k2] - ecol[cc])
-                             : 0.0f;
+        pb[i] = (cc < ncols) ? Bf[(kc + k2) + (int64_t)cc * b] : 0.0f;
       }
     };
     auto stash = [&](int buf) {
@@ -1631,37 +1756,9 @@ __global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ pr
         if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
       }
     }
-    // Equilibration (graded panels: H(i, j) 2^(c_j - r_i) and B(j, s)
-    // 2^(-c_j - e_s) are of order one wherever H = D H0 D^-1 with a diagonal
-    // D of powers of two, however wide its range; the product is scaled back
-    // by 2^(r_i + e_s) in FP64).  1: column exponents over the block's rows.
-    for (int j = wave; j < b; j += 4) {
-      const double a = wave_max(lane < nr ? fabs(Hp[(r0 + lane) + (int64_t)j * ldH]) : 0.0);
-      if (lane == 0) ccol[j] = (a > 0.0 && isfinite(a)) ? -ilogb(a) : 0;
-    }
-    __syncthreads();
-    {   // 2: row exponents of the column-scaled panel
-      const int row = tid & 63, grp = tid >> 6;
-      double a = 0.0;
-      if (row < nr)
-        for (int j = grp; j < b; j += 4)
-          a = fmax(a, ldexp(fabs(Hp[(r0 + row) + (int64_t)j * ldH]), ccol[j]));
-      rmx[grp][row] = a;
-    }
-    __syncthreads();
-    if (tid < GM) {
-      const double a = fmax(fmax(rmx[0][tid], rmx[1][tid]), fmax(rmx[2][tid], rmx[3][tid]));
-      rrow[tid] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
-    }
-    // 3: exponents of the scaled [W | Z] columns
-    for (int cc = wave * (GN / 4); cc < (wave + 1) * (GN / 4); ++cc) {
-      double a = 0.0;
-      if (cc < ncols)
-        for (int j = lane; j < b; j += 64)
-          a = fmax(a, ldexp(fabs(Bg[j + (int64_t)(col0 + cc) * b]), -ccol[j]));
-      a = wave_max(a);
-      if (lane == 0) ecol[cc] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
-    }
+    // Scale-back exponents of this block's rows and [W | Z] columns.
+    if (tid < GM) rrow[tid] = tid < nr ? P.rrow[r0 + tid] : 0;
+    if (tid < GN) ecol[tid] = (col0 + tid) < 2 * m ? P.Be[J & 1][col0 + tid] : 0;
     __syncthreads();
     fetch(0);
     stash(0);
@@ -2034,6 +2131,17 @@ hipError_t launch_norms(const ProbDev* d_probs, const GemmItem* part_items, int 
 }
 
 bool norm_block_needed_host(const ProbDev& P, int rb, int cb) { return norm_block_needed(P, rb, cb); }
+
+hipError_t launch_equilibrate(const ProbDev* d_prob, int n, int nrb, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  // the row pass keeps its partial maxima in the norms' part buffer (nrb x n)
+  const int chunks = nrb < 16 ? nrb : 16;
+  hipLaunchKernelGGL(k_eq_col, dim3((n + 3) / 4), dim3(256), 0, st, d_prob);
+  hipLaunchKernelGGL(k_eq_row, dim3((n + 63) / 64, chunks), dim3(256), 0, st, d_prob);
+  hipLaunchKernelGGL(k_eq_rowfin, dim3((n + 255) / 256), dim3(256), 0, st, d_prob, chunks);
+  hipLaunchKernelGGL(k_eq_conv, dim3((n + 3) / 4), dim3(256), 0, st, d_prob);
+  return hipGetLastError();
+}
 
 hipError_t launch_init(const ProbDev* d_probs, const RowItem* items, int count, hipStream_t st) {
   if (count <= 0) return hipSuccess;

@@ -42,23 +42,26 @@ int choose_pack(int requested, int b, int total_shifts) {
 
 // Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 136
 // VGPRs per wave and 40 KiB of LDS per block; k_diag takes up to 200 VGPRs
-// and 68 KiB, so two panel-pass blocks fit beside it.  The packed sweeps run
-// where many shifts fill the GPU by themselves: there a capped panel pass
-// only slows the plan (256-problem batch in one lane: 31.4 ms without, 41.7
-// ms with one block per CU), so the automatic choice is no lookahead.
-int lookahead_blocks_per_cu(int pack, int b) {
+// and 68 KiB, so two panel-pass blocks fit beside one sweep block.  With more
+// than one sweep block per CU (above 4 shifts x 256 CUs) no panel-pass block
+// fits beside the sweeps: the lookahead then only delays the passes (n =
+// 20000, m = 2000: 94.0 ms with 2 blocks per CU, 64.3 ms without).  The
+// packed sweeps run where many shifts fill the GPU by themselves: there a
+// capped panel pass only slows the plan (256-problem batch in one lane: 31.4
+// ms without, 41.7 ms with one block per CU).  Otherwise no lookahead.
+int lookahead_blocks_per_cu(int pack, int b, int shifts, int cus) {
   (void)b;
-  return pack == 1 ? 2 : 0;
+  return pack == 1 && shifts <= kShiftsPerK1 * std::max(cus, 1) ? 2 : 0;
 }
 
 namespace {
 struct Lay {
   size_t rt0, rt1, c, s, bm0, bm1, aux0, aux1, ynb0, ynb1, rowinf, diagn, part, scal, ep0, ep1,
-      ef, st;
+      ef, st, ccol, rrow, be0, be1, h32, b320, b321;
   size_t total;
 };
 
-Lay layout(int n, int m, int b) {
+Lay layout(int n, int m, int b, bool eq) {
   const int T = (n + b - 1) / b;
   const int b0 = n - (T - 1) * b;
   const int nrb = (b0 + kRowBlock - 1) / kRowBlock + (T - 1) * (b / kRowBlock);
@@ -89,15 +92,25 @@ Lay layout(int n, int m, int b) {
   L.ep1 = take(Tm * sizeof(int));
   L.ef = take(Tm * sizeof(int));
   L.st = take((size_t)std::max(m, 1) * sizeof(int));
+  L.ccol = take((size_t)n * sizeof(int));
+  L.rrow = take((size_t)n * sizeof(int));
+  L.be0 = take((size_t)std::max(2 * m, 1) * sizeof(int));
+  L.be1 = take((size_t)std::max(2 * m, 1) * sizeof(int));
+  L.h32 = L.b320 = L.b321 = 0;
+  if (eq) {   // FP32 operands of the reduced-precision panel pass
+    L.h32 = take((size_t)n * n * sizeof(float));
+    L.b320 = take((size_t)b * 2 * m * sizeof(float));
+    L.b321 = take((size_t)b * 2 * m * sizeof(float));
+  }
   L.total = x;
   return L;
 }
 }  // namespace
 
-size_t prob_workspace_bytes(int n, int m, int b) { return layout(n, m, b).total; }
+size_t prob_workspace_bytes(int n, int m, int b, bool eq) { return layout(n, m, b, eq).total; }
 
 void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact, char* ws,
-                   const int* its_k) {
+                   const int* its_k, bool eq) {
   std::memset(&P, 0, sizeof(P));
   P.H = q.H;
   P.ldH = q.ldH;
@@ -114,7 +127,7 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.its = its;
   P.its_k = its_k;
   P.force_exact = exact;
-  const Lay L = layout(q.n, q.m, b);
+  const Lay L = layout(q.n, q.m, b, eq);
   P.Rt[0] = (double*)(ws + L.rt0);
   P.Rt[1] = (double*)(ws + L.rt1);
   P.C = (double*)(ws + L.c);
@@ -133,6 +146,15 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
   P.Ep[1] = (int*)(ws + L.ep1);
   P.Ef = (int*)(ws + L.ef);
   P.status = (int*)(ws + L.st);
+  if (eq) {
+    P.ccol = (int*)(ws + L.ccol);
+    P.rrow = (int*)(ws + L.rrow);
+    P.Be[0] = (int*)(ws + L.be0);
+    P.Be[1] = (int*)(ws + L.be1);
+    P.H32 = (float*)(ws + L.h32);
+    P.B32[0] = (float*)(ws + L.b320);
+    P.B32[1] = (float*)(ws + L.b321);
+  }
 }
 
 // Tile of the automatic policy.  A single problem below n = 4096 takes

@@ -124,9 +124,9 @@ void plan_free(hsrq_plan* p) {
 }
 
 // Workgroups per CU of the capped panel pass beside a diagonal sweep
-// (lookahead_blocks_per_cu).  Measured, n = 8000, m = 800, tile 128: 9.16 ms
-// without lookahead, 8.83 / 7.60 / 8.87 / 8.58 ms with 1 / 2 / 3 / 4 blocks
-// per CU (MI355X).
+// (lookahead_blocks_per_cu).  Measured, n = 8000, m = 800, tile 128 (MI355X,
+// 128-VGPR sweep and panel pass): 7.9 ms without lookahead, 6.61 / 6.31 /
+// 7.33 ms with 2 / 3 / 4 blocks per CU.
 
 // Item tables of one lane, uploaded once.  `single`: the plan has one lane,
 // which then may take a second stream for the panel pass of the rows above

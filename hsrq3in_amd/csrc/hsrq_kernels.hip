@@ -427,12 +427,16 @@ __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
   return pick_tree<NS>(a, q, NS / 2);
 }
 
+// 128-row tiles: at most 128 VGPRs and 16-column H chunks (33 KiB of LDS),
+// so that three panel-pass workgroups of at most 128 VGPRs and 40 KiB run
+// beside a sweep workgroup on every CU (n = 8000, m = 800: 6.56 -> 6.31 ms
+// against 64-column chunks, up to 193 VGPRs and two panel-pass workgroups).
 template <int RPL>
-__global__ __launch_bounds__(256) void k_diag(const ProbDev* __restrict__ probs,
-                                              const K1Item* __restrict__ items) {
+__global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* __restrict__ probs,
+                                                              const K1Item* __restrict__ items) {
   __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   constexpr int BMAX = 64 * RPL;
-  constexpr int CW = (RPL == 4) ? 16 : 64;   // tile columns per LDS chunk
+  constexpr int CW = (RPL >= 2) ? 16 : 64;   // tile columns per LDS chunk
   __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
   __shared__ double Bsh[CW * 2 * 4];         // [W | Z] chunk of the 4 shifts
 
@@ -1440,9 +1444,11 @@ constexpr int LDB64 = KC + 2;
 constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 
 // Block = one 64-row block x 64 columns of [W | Z] (32 shifts); four waves,
-// 32 rows x 32 columns each (2 x 2 MFMA tiles).
+// 32 rows x 32 columns each (2 x 2 MFMA tiles).  At most 128 VGPRs: four
+// blocks per CU by registers and LDS (the 256-problem batch: 22.0 -> 21.4
+// ms), three beside a sweep workgroup.
 template <bool LOOP>
-__global__ __launch_bounds__(256) void k_update(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ probs,
                                                 const GemmItem* __restrict__ items, int count) {
   __shared__ double As[2][KC * LDA64];   // As[buf][k * LDA64 + row]
   __shared__ double Bs[2][GN * LDB64];   // Bs[buf][col * LDB64 + k]

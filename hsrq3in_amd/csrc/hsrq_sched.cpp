@@ -40,9 +40,11 @@ int choose_pack(int requested, int b, int total_shifts) {
   return total_shifts >= kPackedMinShifts ? 4 : 1;
 }
 
-// Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 136
-// VGPRs per wave and 40 KiB of LDS per block; k_diag takes up to 200 VGPRs
-// and 68 KiB, so two panel-pass blocks fit beside one sweep block.  With more
+// Per SIMD 512 VGPRs, per CU 160 KiB LDS.  The looping panel pass takes 128
+// VGPRs per wave and 40 KiB of LDS per block; the 128-row k_diag 128 VGPRs
+// and 33 KiB, so three panel-pass blocks fit beside one sweep block (n =
+// 8000, m = 800: 6.56 ms with two blocks per CU, 6.31 ms with three; with
+// four the sweeps wait for room: 7.33 ms).  With more
 // than one sweep block per CU (above 4 shifts x 256 CUs) no panel-pass block
 // fits beside the sweeps: the lookahead then only delays the passes (n =
 // 20000, m = 2000: 94.0 ms with 2 blocks per CU, 64.3 ms without).  The
@@ -51,7 +53,7 @@ int choose_pack(int requested, int b, int total_shifts) {
 // ms without, 41.7 ms with one block per CU).  Otherwise no lookahead.
 int lookahead_blocks_per_cu(int pack, int b, int shifts, int cus) {
   (void)b;
-  return pack == 1 && shifts <= kShiftsPerK1 * std::max(cus, 1) ? 2 : 0;
+  return pack == 1 && shifts <= kShiftsPerK1 * std::max(cus, 1) ? 3 : 0;
 }
 
 namespace {

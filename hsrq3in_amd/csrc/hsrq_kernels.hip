@@ -1666,7 +1666,7 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
 // so a lane holds one row and four adjacent columns, W (even) and Z (odd).
 // ---------------------------------------------------------------------------
 template <bool LOOP>
-__global__ __launch_bounds__(256) void k_update32(const ProbDev* __restrict__ probs,
+__global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__ probs,
                                                   const GemmItem* __restrict__ items, int count) {
   __shared__ float As[2][KC * LDA];
   __shared__ float Bs[2][KC * LDB];

@@ -36,12 +36,12 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per k_update launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (calibrated against a copy of known size), by (n, m, tile).
 K_UPDATE_TRAFFIC = {
-    (8000, 800, 128): (140.94e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
+    (8000, 800, 128): (136.76e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
 }
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_diag launch (same PMC runs), by (n, m, tile).
 K_DIAG_TRAFFIC = {
-    (8000, 800, 128): (10.01e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
+    (8000, 800, 128): (14.05e6, "profiles/traffic_plain_n8000_m800_tile128.md"),
 }
 
 

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_norm_part(const ProbDev* __restrict__ p
   double a = 0.0;
   if (row < nr) {
     const int i = r0 + row;
-    const double* h = P.H + i;
+    const auto* h = gm(P.H + i);
     double v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_norm_part(const ProbDev* __restrict__ p
   red[g][row] = a;
   __syncthreads();
   if (threadIdx.x < nr)
-    P.part[(int64_t)it.cb * P.n + r0 + threadIdx.x] =
+    gm(P.part)[(int64_t)it.cb * P.n + r0 + threadIdx.x] =
         (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
 }
 
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_norm_final(const ProbDev* __restrict__ 
     // loaded 8 at a time (independent loads in flight), added in cb order
     int cb_lo = 0;
     while (cb_lo < P.nrb && !norm_block_needed(P, rb, cb_lo)) ++cb_lo;
-    const double* __restrict__ pc = P.part + l + r;
+    const auto* __restrict__ pc = gm(P.part + l + r);
     for (int cb0 = cb_lo; cb0 < P.nrb; cb0 += 8) {
       double v[8];
 #pragma unroll
@@ -114,8 +114,8 @@ __global__ __launch_bounds__(256) void k_norm_final(const ProbDev* __restrict__ 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    P.rowinf[I] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-    P.diagn[I] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+    gm(P.rowinf)[I] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    gm(P.diagn)[I] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
   }
 }
 
@@ -132,12 +132,12 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
   const int k = it.k0 + wave;
   if (k >= P.m) return;  // whole wavefront exits; no barriers below
   double hinf = 0.0;
-  for (int t = lane; t < P.T; t += 64) hinf = fmax(hinf, P.rowinf[t]);
+  for (int t = lane; t < P.T; t += 64) hinf = fmax(hinf, gm(P.rowinf)[t]);
   hinf = wave_max(hinf);
   const double eps3 = hinf * 0x1p-53;   // inverse_iteration.c:157-158
   if (it.rb == 0 && k == 0 && lane == 0) {
-    P.scal[0] = hinf;
-    P.scal[1] = eps3;
+    gm(P.scal)[0] = hinf;
+    gm(P.scal)[1] = eps3;
   }
   int tile, r0, nr;
   rowblock_range(P, it.rb, tile, r0, nr);
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
   if (lane < nr) {
     const int row = r0 + lane;
     if (its == -2) {
-      yv = P.Y[row + (int64_t)k * P.ldY];   // given right-hand side
+      yv = gm(P.Y)[row + (int64_t)k * P.ldY];   // given right-hand side
     } else if (its < 0) {
       yv = eps3;
     } else {
@@ -155,15 +155,15 @@ __global__ __launch_bounds__(256) void k_init(const ProbDev* __restrict__ probs,
       yv = (row == 0) ? eps3 : eps3 / (sq + 1.0);
       if (row == n - 1 - its) yv -= eps3 * sq;
     }
-    if (its != -2) P.Y[row + (int64_t)k * P.ldY] = yv;
+    if (its != -2) gm(P.Y)[row + (int64_t)k * P.ldY] = yv;
     double h = P.H[row + (int64_t)(n - 1) * P.ldH];
-    if (row == n - 1) h -= P.wr[k];
-    P.Rt[(P.T - 1) & 1][row + (int64_t)k * n] = h;
+    if (row == n - 1) h -= gm(P.wr)[k];
+    gm(P.Rt[(P.T - 1) & 1])[row + (int64_t)k * n] = h;
   }
   const double mx = wave_max(fabs(yv));
-  if (lane == 0) P.Ynb[P.T & 1][(int64_t)it.rb * P.m + k] = mx;
+  if (lane == 0) gm(P.Ynb[P.T & 1])[(int64_t)it.rb * P.m + k] = mx;
   if (it.rb == 0) {
-    for (int t = lane; t < P.T; t += 64) P.Ep[P.T & 1][(int64_t)t * P.m + k] = 0;
+    for (int t = lane; t < P.T; t += 64) gm(P.Ep[P.T & 1])[(int64_t)t * P.m + k] = 0;
     if (lane == 0) P.status[k] = 0;
   }
 }
@@ -182,11 +182,11 @@ __global__ __launch_bounds__(256) void k_eq_col(const ProbDev* __restrict__ prob
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= P.n) return;   // whole wavefront
   const int rows = min(j + 2, P.n);
-  const double* __restrict__ col = P.H + (int64_t)j * P.ldH;
+  const auto* __restrict__ col = gm(P.H + (int64_t)j * P.ldH);
   double a = 0.0;
   for (int i = lane; i < rows; i += 64) a = fmax(a, fabs(col[i]));
   a = wave_max(a);
-  if (lane == 0) P.ccol[j] = (a > 0.0 && isfinite(a)) ? -ilogb(a) : 0;
+  if (lane == 0) gm(P.ccol)[j] = (a > 0.0 && isfinite(a)) ? -ilogb(a) : 0;
 }
 
 // Row maxima in two passes: block (64 rows, column chunk blockIdx.y of
@@ -202,11 +202,11 @@ __global__ __launch_bounds__(256) void k_eq_row(const ProbDev* __restrict__ prob
   const int ja = j0 + blockIdx.y * len, jb = min(ja + len, P.n);
   double a = 0.0;
   for (int j = ja + ph; j < jb; j += 4)
-    if (i < P.n && j >= i - 1) a = fmax(a, ldexp(fabs(P.H[i + (int64_t)j * ldH]), P.ccol[j]));
+    if (i < P.n && j >= i - 1) a = fmax(a, ldexp(fabs(P.H[i + (int64_t)j * ldH]), gm(P.ccol)[j]));
   red[ph][r] = a;
   __syncthreads();
   if (ph == 0 && i < P.n)
-    P.part[(int64_t)blockIdx.y * P.n + i] =
+    gm(P.part)[(int64_t)blockIdx.y * P.n + i] =
         fmax(fmax(red[0][r], red[1][r]), fmax(red[2][r], red[3][r]));
 }
 
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256) void k_eq_rowfin(const ProbDev* __restrict__ p
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= P.n) return;
   double a = 0.0;
-  for (int c = 0; c < chunks; ++c) a = fmax(a, P.part[(int64_t)c * P.n + i]);
-  P.rrow[i] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
+  for (int c = 0; c < chunks; ++c) a = fmax(a, gm(P.part)[(int64_t)c * P.n + i]);
+  gm(P.rrow)[i] = (a > 0.0 && isfinite(a)) ? ilogb(a) : 0;
 }
 
 // H32(i, j) = H(i, j) 2^(ccol[j] - rrow[i]) in float over the Hessenberg part
@@ -226,10 +226,10 @@ __global__ __launch_bounds__(256) void k_eq_conv(const ProbDev* __restrict__ pro
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= P.n) return;
   const int rows = min(j + 2, P.n);
-  const double* __restrict__ col = P.H + (int64_t)j * P.ldH;
-  float* __restrict__ out = P.H32 + (int64_t)j * P.n;
-  const int cj = P.ccol[j];
-  for (int i = lane; i < rows; i += 64) out[i] = (float)ldexp(col[i], cj - P.rrow[i]);
+  const auto* __restrict__ col = gm(P.H + (int64_t)j * P.ldH);
+  auto* __restrict__ out = gm(P.H32 + (int64_t)j * P.n);
+  const int cj = gm(P.ccol)[j];
+  for (int i = lane; i < rows; i += 64) out[i] = (float)ldexp(col[i], cj - gm(P.rrow)[i]);
 }
 
 // Exponent of the largest |x_r| 2^-ccol[l - 1 + r] over a shift's coefficient
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_eq_conv(const ProbDev* __restrict__ pro
 // of the [W | Z] column in the FP32 panel pass.  Per lane: its rows' best.
 __device__ __forceinline__ double coef_exp_part(const ProbDev& P, int l, double x, int r, int nb,
                                                 double cur) {
-  if (r < nb && x != 0.0) cur = fmax(cur, (double)(ilogb(x) - P.ccol[l - 1 + r]));
+  if (r < nb && x != 0.0) cur = fmax(cur, (double)(ilogb(x) - gm(P.ccol)[l - 1 + r]));
   return cur;
 }
 
@@ -255,18 +255,18 @@ __device__ __forceinline__ double coef_exp_part(const ProbDev& P, int l, double 
 __device__ __forceinline__ int update_exponents(const ProbDev& P, int I, int J, int k, double mu,
                                                 int& sy, int& sz) {
   const int m = P.m;
-  const double* __restrict__ Yb = P.Ynb[(J + 1) & 1];
+  const auto* __restrict__ Yb = gm(P.Ynb[(J + 1) & 1]);
   const int rb0 = tile_first_rowblock(P, I), nrbI = tile_num_rowblocks(P, I);
   double yn = 0.0;
   for (int rb = rb0; rb < rb0 + nrbI; ++rb) yn = fmax(yn, Yb[(int64_t)rb * m + k]);
-  const int EI = P.Ep[(J + 1) & 1][(int64_t)I * m + k];
-  const int EJ = P.Ef[(int64_t)J * m + k];
-  const double zn = P.aux[J & 1][4 * k + 2];
-  const int p = (int)P.aux[J & 1][4 * k + 3];
+  const int EI = gm(P.Ep[(J + 1) & 1])[(int64_t)I * m + k];
+  const int EJ = gm(P.Ef)[(int64_t)J * m + k];
+  const double zn = gm(P.aux[J & 1])[4 * k + 2];
+  const int p = (int)gm(P.aux[J & 1])[4 * k + 3];
   const int c0 = min(EI, EJ);
   const double yI = ldexp(yn, c0 - EI);
   const double zJ = ldexp(zn, c0 - EJ);
-  const double tI = 2.0 * (P.rowinf[I] + fabs(mu));
+  const double tI = 2.0 * (gm(P.rowinf)[I] + fabs(mu));
   const int ec = c0 + pow2_exp(protect_update(tI, zJ, yI));
   sy = ec - EI;
   sz = ec - EJ + p;
@@ -285,16 +285,16 @@ __device__ __forceinline__ int update_exponents(const ProbDev& P, int I, int J, 
 // Lanes hold rows r0 + STRIDE q.  Every sweep layout computes each row the
 // same way, so the layouts stay bit-identical.
 // ---------------------------------------------------------------------------
-template <int RPL, int STRIDE, int CW, int BMAX, int NL, int SPW>
+template <int RPL, int STRIDE, int CW, int BMAX, int NL, int SPW, bool GLOBAL>
 __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0, int k, int sidx,
                                                 int l, int nb, int r0, double mu, double* Hs,
                                                 double* Bsh, double (&pf)[NL], double (&v)[RPL],
                                                 double (&y)[RPL], int& E0) {
   const int b = P.b, m = P.m, n = P.n;
   // epilogue operands first: their loads overlap the panel product
-  const double* aux = P.aux[(J + 1) & 1];
-  const double* __restrict__ Rin = P.Rt[(J + 1) & 1] + (int64_t)k * n + l;
-  const double* __restrict__ yg = P.Y + (int64_t)k * P.ldY + l;
+  const auto* aux = gsel<GLOBAL>(P.aux[(J + 1) & 1]);
+  const auto* __restrict__ Rin = gsel<GLOBAL>(P.Rt[(J + 1) & 1] + (int64_t)k * n + l);
+  const auto* __restrict__ yg = gsel<GLOBAL>(P.Y + (int64_t)k * P.ldY + l);
   double rin[RPL], yo[RPL];
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
@@ -302,11 +302,11 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
     rin[q] = r < nb ? Rin[r] : 0.0;
     yo[q] = r < nb ? yg[r] : 0.0;
   }
-  const double* __restrict__ Bg = P.Bm[(J + 1) & 1];
+  const auto* __restrict__ Bg = gsel<GLOBAL>(P.Bm[(J + 1) & 1]);
   const double Pk = aux[4 * k + 0], Zl = aux[4 * k + 1];
   const double b0w = Bg[(int64_t)(2 * k) * b], b0z = Bg[(int64_t)(2 * k + 1) * b];
   const int64_t ldH = P.ldH;
-  const double* __restrict__ Hq = P.H + (int64_t)(l + nb - 1) * ldH + l;
+  const auto* __restrict__ Hq = gsel<GLOBAL>(P.H + (int64_t)(l + nb - 1) * ldH + l);
   // [W | Z] of the workgroup's SPW shifts, CW columns per chunk, in LDS as
   // Bsh[c * 2 SPW + 2 s + {0: W, 1: Z}]
   constexpr int NBT = CW * 2 * SPW, NB = (NBT + 255) / 256;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void own_rows_update(const ProbDev& P, int J, int k0,
   const int hr = threadIdx.x % BMAX, hcb = threadIdx.x / BMAX;
   const int64_t hstep = (int64_t)CSTEP * ldH;
   auto stage = [&](int chunk) __attribute__((always_inline)) {
-    const double* __restrict__ src = Hq + hr + (int64_t)(chunk * CW + hcb) * ldH;
+    const auto* __restrict__ src = Hq + hr + (int64_t)(chunk * CW + hcb) * ldH;
     if (nb == BMAX) {   // full tile
 #pragma unroll
       for (int i = 0; i < NL; ++i) pf[i] = src[i * hstep];
@@ -449,11 +449,11 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
   const bool valid = kraw < m;
   const int k = valid ? kraw : m - 1;
   const int l = tile_first_row(P, J), nb = tile_rows(P, J);
-  const double mu = P.wr[k];
-  const double* __restrict__ H = P.H;
+  const double mu = gm(P.wr)[k];
+  const auto* __restrict__ H = gm(P.H);
   const int64_t ldH = P.ldH;
-  const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
-  double* yg = P.Y + (int64_t)k * P.ldY + l;
+  const auto* Rin = gm(P.Rt[J & 1] + (int64_t)k * n + l);
+  auto* yg = gm(P.Y + (int64_t)k * P.ldY + l);
 
   // H tile chunks: CW columns each, staged through registers one chunk ahead
   // so the global loads overlap the sweep of the previous chunk.
@@ -467,7 +467,7 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
     const int r = threadIdx.x % BMAX, cb = threadIdx.x / BMAX;
-    const double* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
+    const auto* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
     const int64_t step = (int64_t)CSTEP * ldH;
     if (nc == CW && nb == BMAX) {   // full chunk of a full tile
 #pragma unroll
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
   double v[RPL], y[RPL], cr[RPL], sr[RPL];
   int E;   // scale exponent of the tile's y
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
-    own_rows_update<RPL, 64, CW, BMAX, NL, 4>(P, J, it.k0, k, wave, l, nb, lane, mu, Hs, Bsh, pf,
+    own_rows_update<RPL, 64, CW, BMAX, NL, 4, true>(P, J, it.k0, k, wave, l, nb, lane, mu, Hs, Bsh, pf,
                                              v, y, E);
   } else {       // the last tile column starts from the initial vectors
 #pragma unroll
@@ -490,14 +490,14 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
       v[q] = r < nb ? Rin[r] : 0.0;
       y[q] = r < nb ? yg[r] : 0.0;
     }
-    E = P.Ep[(J + 1) & 1][(int64_t)J * m + k];
+    E = gm(P.Ep[(J + 1) & 1])[(int64_t)J * m + k];
   }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     cr[q] = 1.0;
     sr[q] = 0.0;
   }
-  const double Hnorm = P.diagn[J];
+  const double Hnorm = gm(P.diagn)[J];
   const double eps = 0x1p-53;
   const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
   const double smin = fmax(eps * fabs(mu), smlnum);                        // solve-task.c:47
@@ -785,8 +785,8 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
   }
 
   if (valid) {
-    double* Cg = P.C + (int64_t)k * n + l;
-    double* Sg = P.S + (int64_t)k * n + l;
+    auto* Cg = gm(P.C + (int64_t)k * n + l);
+    auto* Sg = gm(P.S + (int64_t)k * n + l);
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = lane + 64 * q;
@@ -797,7 +797,7 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
       }
     }
     if (lane == 0) {
-      P.Ef[(int64_t)J * m + k] = E;
+      gm(P.Ef)[(int64_t)J * m + k] = E;
       if (info) atomicOr(&P.status[k], 2);
     }
   }
@@ -868,7 +868,7 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
 
   // Prescale so that the panel product cannot overflow:
   // |acc| <= 2 (||H||_oo + |mu|) ||Z||_oo must stay below 2^1020.
-  const double hinf = P.scal[0];
+  const double hinf = gm(P.scal)[0];
   const double tmax = 2.0 * (hinf + fabs(mu));
   int p = 0;
   if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
@@ -885,15 +885,15 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
     const int iw = ew > -1e300 ? (int)ew : 0, iz = ez > -1e300 ? (int)ez : 0;
     if (valid) {
       if (lane == 0) {
-        P.Be[J & 1][2 * k] = iw;
-        P.Be[J & 1][2 * k + 1] = iz;
+        gm(P.Be[J & 1])[2 * k] = iw;
+        gm(P.Be[J & 1])[2 * k + 1] = iz;
       }
-      float* Fw = P.B32[J & 1] + (int64_t)(2 * k) * P.b;
+      auto* Fw = gm(P.B32[J & 1] + (int64_t)(2 * k) * P.b);
 #pragma unroll
       for (int q = 0; q < RPL; ++q) {
         const int r = lane + 64 * q;
         if (r < nb) {
-          const int cr = P.ccol[l - 1 + r];
+          const int cr = gm(P.ccol)[l - 1 + r];
           Fw[r] = (float)ldexp(w[q], -cr - iw);
           Fw[P.b + r] = (float)ldexp(ldexp(z[q], -p), -cr - iz);
         }
@@ -901,8 +901,8 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
     }
   }
   if (valid) {
-    double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
-    double* Bz = Bw + P.b;
+    auto* Bw = gm(P.Bm[J & 1] + (int64_t)(2 * k) * P.b);
+    auto* Bz = Bw + P.b;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = lane + 64 * q;
@@ -912,10 +912,10 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
       }
     }
     if (lane == 0) {
-      P.aux[J & 1][4 * k + 0] = Pk;
-      P.aux[J & 1][4 * k + 1] = ldexp(zlast, -p);
-      P.aux[J & 1][4 * k + 2] = zn;
-      P.aux[J & 1][4 * k + 3] = (double)p;
+      gm(P.aux[J & 1])[4 * k + 0] = Pk;
+      gm(P.aux[J & 1])[4 * k + 1] = ldexp(zlast, -p);
+      gm(P.aux[J & 1])[4 * k + 2] = zn;
+      gm(P.aux[J & 1])[4 * k + 3] = (double)p;
     }
   }
 }
@@ -957,11 +957,11 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   const bool valid = kraw < m;
   const int k = valid ? kraw : m - 1;
   const int l = tile_first_row(P, J), nb = tile_rows(P, J);
-  const double mu = P.wr[k];
-  const double* __restrict__ H = P.H;
+  const double mu = gsel<false>(P.wr)[k];
+  const auto* __restrict__ H = gsel<false>(P.H);
   const int64_t ldH = P.ldH;
-  const double* Rin = P.Rt[J & 1] + (int64_t)k * n + l;
-  double* yg = P.Y + (int64_t)k * P.ldY + l;
+  const auto* Rin = gsel<false>(P.Rt[J & 1] + (int64_t)k * n + l);
+  auto* yg = gsel<false>(P.Y + (int64_t)k * P.ldY + l);
 
   constexpr int NL = CW * BMAX / 256;
   double pf[NL];
@@ -973,7 +973,7 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     const int c0 = chunk * CW;
     const int nc = min(CW, nb - c0);
     const int r = threadIdx.x % BMAX, cb = threadIdx.x / BMAX;
-    const double* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
+    const auto* __restrict__ src = H + (l + r) + (int64_t)(l + c0 + cb) * ldH;
     const int64_t step = (int64_t)CSTEP * ldH;
     if (nc == CW && nb == BMAX) {   // full chunk of a full tile
 #pragma unroll
@@ -988,11 +988,11 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   // rotations go to C, S as they are made (the lane owning the row stores
   // them), not kept in registers: 2 RPL registers and the selects of a
   // register-array update per step less
-  double* __restrict__ Cg = P.C + (int64_t)k * n + l;
-  double* __restrict__ Sg = P.S + (int64_t)k * n + l;
+  auto* __restrict__ Cg = gsel<false>(P.C + (int64_t)k * n + l);
+  auto* __restrict__ Sg = gsel<false>(P.S + (int64_t)k * n + l);
   int E;   // scale exponent of the tile's y
   if (J + 1 < P.T) {   // rows of this tile after the panel pass of tile column J+1
-    own_rows_update<RPL, 16, CW, BMAX, NL, 16>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
+    own_rows_update<RPL, 16, CW, BMAX, NL, 16, false>(P, J, it.k0, k, wave * 4 + grp, l, nb, t, mu, Hs,
                                                Bsh, pf, v, y, E);
   } else {       // the last tile column starts from the initial vectors
 #pragma unroll
@@ -1001,9 +1001,9 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
       v[q] = r < nb ? Rin[r] : 0.0;
       y[q] = r < nb ? yg[r] : 0.0;
     }
-    E = P.Ep[(J + 1) & 1][(int64_t)J * m + k];
+    E = gsel<false>(P.Ep[(J + 1) & 1])[(int64_t)J * m + k];
   }
-  const double Hnorm = P.diagn[J];
+  const double Hnorm = gsel<false>(P.diagn)[J];
   const double eps = 0x1p-53;
   const double smlnum = fmax(2 * DBL_MIN, DBL_MIN * ((double)n / eps));   // s_H_s.c:70-71
   const double smin = fmax(eps * fabs(mu), smlnum);                        // solve-task.c:47
@@ -1318,7 +1318,7 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
       if (r < nb) yg[r] = y[q];
     }
     if (t == 0) {
-      P.Ef[(int64_t)J * m + k] = E;
+      gsel<false>(P.Ef)[(int64_t)J * m + k] = E;
       if (info) atomicOr(&P.status[k], 2);
     }
   }
@@ -1372,7 +1372,7 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
   const double Pk = carry_p;
   const double zlast = carry_t;
   const double zn = fmax(row16_max(zmax), fabs(zlast));
-  const double hinf = P.scal[0];
+  const double hinf = gsel<false>(P.scal)[0];
   const double tmax = 2.0 * (hinf + fabs(mu));
   int p = 0;
   if (zn > 0.0 && tmax > 0.0) p = max(0, ilogb(tmax) + ilogb(zn) + 2 - 1020);
@@ -1389,15 +1389,15 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     const int iw = ew > -1e300 ? (int)ew : 0, iz = ez > -1e300 ? (int)ez : 0;
     if (valid) {
       if (t == 0) {
-        P.Be[J & 1][2 * k] = iw;
-        P.Be[J & 1][2 * k + 1] = iz;
+        gsel<false>(P.Be[J & 1])[2 * k] = iw;
+        gsel<false>(P.Be[J & 1])[2 * k + 1] = iz;
       }
-      float* Fw = P.B32[J & 1] + (int64_t)(2 * k) * P.b;
+      auto* Fw = gsel<false>(P.B32[J & 1] + (int64_t)(2 * k) * P.b);
 #pragma unroll
       for (int q = 0; q < RPL; ++q) {
         const int r = t + 16 * q;
         if (r < nb) {
-          const int cr = P.ccol[l - 1 + r];
+          const int cr = gsel<false>(P.ccol)[l - 1 + r];
           Fw[r] = (float)ldexp(w[q], -cr - iw);
           Fw[P.b + r] = (float)ldexp(ldexp(z[q], -p), -cr - iz);
         }
@@ -1405,8 +1405,8 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
     }
   }
   if (valid) {
-    double* Bw = P.Bm[J & 1] + (int64_t)(2 * k) * P.b;
-    double* Bz = Bw + P.b;
+    auto* Bw = gsel<false>(P.Bm[J & 1] + (int64_t)(2 * k) * P.b);
+    auto* Bz = Bw + P.b;
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
       const int r = t + 16 * q;
@@ -1416,10 +1416,10 @@ __global__ __launch_bounds__(256, RPL <= 8 ? 3 : 1) void k_diag16(const ProbDev*
       }
     }
     if (t == 0) {
-      P.aux[J & 1][4 * k + 0] = Pk;
-      P.aux[J & 1][4 * k + 1] = ldexp(zlast, -p);
-      P.aux[J & 1][4 * k + 2] = zn;
-      P.aux[J & 1][4 * k + 3] = (double)p;
+      gsel<false>(P.aux[J & 1])[4 * k + 0] = Pk;
+      gsel<false>(P.aux[J & 1])[4 * k + 1] = ldexp(zlast, -p);
+      gsel<false>(P.aux[J & 1])[4 * k + 2] = zn;
+      gsel<false>(P.aux[J & 1])[4 * k + 3] = (double)p;
     }
   }
 }
@@ -1483,8 +1483,8 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t ldH = P.ldH;
 
-    const double* __restrict__ Hp = P.H + (int64_t)(l - 1) * ldH;   // panel column 0 = l-1
-    const double* __restrict__ Bg = P.Bm[J & 1];
+    const auto* __restrict__ Hp = gsel<false>(P.H + (int64_t)(l - 1) * ldH);   // panel column 0 = l-1
+    const auto* __restrict__ Bg = gsel<false>(P.Bm[J & 1]);
 
     // global -> register staging of one K chunk
     constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
@@ -1532,8 +1532,8 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
     if (tid < GN / 2) {
       const int k = col0 / 2 + tid;
       if (k < m) {
-        const double* aux = P.aux[J & 1];
-        const double mu = P.wr[k];
+        const auto* aux = gsel<false>(P.aux[J & 1]);
+        const double mu = gsel<false>(P.wr)[k];
         pmu[tid] = mu;
         pP[tid] = aux[4 * k + 0];
         pZl[tid] = aux[4 * k + 1];
@@ -1543,7 +1543,7 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
         const int ec = update_exponents(P, tile, J, k, mu, sy, sz);
         psy[tid] = sy;
         psz[tid] = sz;
-        if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
+        if (it.rb == tile_first_rowblock(P, tile)) gsel<false>(P.Ep[J & 1])[(int64_t)tile * m + k] = ec;
       }
     }
     stash(0);
@@ -1553,8 +1553,8 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
     // acc[a][c][r] covers row wm*32 + a*16 + lane%16 and column
     // wn*GN/2 + c*16 + 4*r + lane/16; even lane groups hold W columns (R~
     // out), odd ones Z columns (y).
-    const double* __restrict__ Rin = P.Rt[J & 1];
-    double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+    const auto* __restrict__ Rin = gsel<false>(P.Rt[J & 1]);
+    auto* __restrict__ Rout = gsel<false>(P.Rt[(J + 1) & 1]);
     const int g = lane >> 4;
     const bool isz = (g & 1) != 0;
     const int rl = lane & 15;
@@ -1571,7 +1571,7 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
             const int gi = r0 + row;
             const bool ok = okc && row < nr;
             rin[r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-            yv[r][a] = (ok && isz) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+            yv[r][a] = (ok && isz) ? gsel<false>(P.Y)[gi + (int64_t)k * P.ldY] : 0.0;
           }
         }
     };
@@ -1629,7 +1629,7 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
                 Rout[gi + (int64_t)k * n] = val;
               } else {
                 const double ynew = ldexp(yv[r][a], sy) - ldexp(val, sz);
-                P.Y[gi + (int64_t)k * P.ldY] = ynew;
+                gsel<false>(P.Y)[gi + (int64_t)k * P.ldY] = ynew;
                 cmax = fmax(cmax, fabs(ynew));
               }
             }
@@ -1644,7 +1644,7 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
       const int cl = tid;
       if (cl < ncols && (cl & 1)) {
         const int k = (col0 + cl) >> 1;
-        P.Ynb[J & 1][(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+        gsel<false>(P.Ynb[J & 1])[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
       }
     }
     if (!LOOP) break;
@@ -1701,11 +1701,11 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
 
-    const double* __restrict__ Bg = P.Bm[J & 1];
+    const auto* __restrict__ Bg = gsel<false>(P.Bm[J & 1]);
     // equilibrated float operands (k_eq_conv, the sweep): panel columns
     // l-1 .. l+b-2, the block's rows; [W | Z] columns col0 ..
-    const float* __restrict__ Hf = P.H32 + (int64_t)(l - 1) * n + r0;
-    const float* __restrict__ Bf = P.B32[J & 1] + (int64_t)col0 * b;
+    const auto* __restrict__ Hf = gsel<false>(P.H32 + (int64_t)(l - 1) * n + r0);
+    const auto* __restrict__ Bf = gsel<false>(P.B32[J & 1] + (int64_t)col0 * b);
 
     constexpr int NPA = KC * GM / 256, NPB = KC * GN / 256;
     float pa[NPA], pb[NPB];
@@ -1748,8 +1748,8 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
     if (tid < GN / 2) {
       const int k = col0 / 2 + tid;
       if (k < m) {
-        const double* aux = P.aux[J & 1];
-        const double mu = P.wr[k];
+        const auto* aux = gsel<false>(P.aux[J & 1]);
+        const double mu = gsel<false>(P.wr)[k];
         pmu[tid] = mu;
         pP[tid] = aux[4 * k + 0];
         pZl[tid] = aux[4 * k + 1];
@@ -1759,12 +1759,12 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
         const int ec = update_exponents(P, tile, J, k, mu, sy, sz);
         psy[tid] = sy;
         psz[tid] = sz;
-        if (it.rb == tile_first_rowblock(P, tile)) P.Ep[J & 1][(int64_t)tile * m + k] = ec;
+        if (it.rb == tile_first_rowblock(P, tile)) gsel<false>(P.Ep[J & 1])[(int64_t)tile * m + k] = ec;
       }
     }
     // Scale-back exponents of this block's rows and [W | Z] columns.
-    if (tid < GM) rrow[tid] = tid < nr ? P.rrow[r0 + tid] : 0;
-    if (tid < GN) ecol[tid] = (col0 + tid) < 2 * m ? P.Be[J & 1][col0 + tid] : 0;
+    if (tid < GM) rrow[tid] = tid < nr ? gsel<false>(P.rrow)[r0 + tid] : 0;
+    if (tid < GN) ecol[tid] = (col0 + tid) < 2 * m ? gsel<false>(P.Be[J & 1])[col0 + tid] : 0;
     __syncthreads();
     fetch(0);
     stash(0);
@@ -1793,8 +1793,8 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
 
     // Epilogue.  acc[a][c][r]: row wm*32 + a*16 + lane%16,
     //                          col wn*GN/2 + c*16 + 4*(lane/16) + r.
-    const double* __restrict__ Rin = P.Rt[J & 1];
-    double* __restrict__ Rout = P.Rt[(J + 1) & 1];
+    const auto* __restrict__ Rin = gsel<false>(P.Rt[J & 1]);
+    auto* __restrict__ Rout = gsel<false>(P.Rt[(J + 1) & 1]);
     const int g = lane >> 4;
     const int rl = lane & 15;
     double rin[CT][4][2], yv[CT][4][2];
@@ -1811,7 +1811,7 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
           const int gi = r0 + row;
           const bool ok = okc && row < nr;
           rin[c][r][a] = ok ? Rin[gi + (int64_t)k * n] : 0.0;
-          yv[c][r][a] = (ok && (r & 1)) ? P.Y[gi + (int64_t)k * P.ldY] : 0.0;
+          yv[c][r][a] = (ok && (r & 1)) ? gsel<false>(P.Y)[gi + (int64_t)k * P.ldY] : 0.0;
         }
       }
     double cmaxs[CT][2];
@@ -1845,7 +1845,7 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
                 Rout[gi + (int64_t)k * n] = val;
               } else {
                 const double ynew = ldexp(yv[c][r][a], sy) - ldexp(val, sz);
-                P.Y[gi + (int64_t)k * P.ldY] = ynew;
+                gsel<false>(P.Y)[gi + (int64_t)k * P.ldY] = ynew;
                 cmaxs[c][r >> 1] = fmax(cmaxs[c][r >> 1], fabs(ynew));
               }
             }
@@ -1866,7 +1866,7 @@ __global__ __launch_bounds__(256, 4) void k_update32(const ProbDev* __restrict__
       const int cl = tid;
       if (cl < ncols && (cl & 1)) {
         const int k = (col0 + cl) >> 1;
-        P.Ynb[J & 1][(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
+        gsel<false>(P.Ynb[J & 1])[(int64_t)it.rb * m + k] = fmax(red[0][cl], red[1][cl]);
       }
     }
     if (!LOOP) break;
@@ -1908,16 +1908,16 @@ __global__ __launch_bounds__(256) void k_backtransform(const ProbDev* __restrict
 
   // smallest exponent over the tiles (reduce_scaling_factors, robust.c:58-82)
   int em = 0x7fffffff;
-  for (int t = tid; t < T; t += 256) em = min(em, P.Ef[(int64_t)t * m + k]);
+  for (int t = tid; t < T; t += 256) em = min(em, gm(P.Ef)[(int64_t)t * m + k]);
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) em = min(em, __shfl_xor(em, off, 64));
   if (lane == 0) emin_s[wave] = em;
   __syncthreads();
   const int emin = min(min(emin_s[0], emin_s[1]), min(emin_s[2], emin_s[3]));
 
-  double* x = P.Y + (int64_t)k * P.ldY;
-  const double* c = P.C + (int64_t)k * n;
-  const double* s = P.S + (int64_t)k * n;
+  auto* x = gm(P.Y + (int64_t)k * P.ldY);
+  const auto* c = gm(P.C + (int64_t)k * n);
+  const auto* s = gm(P.S + (int64_t)k * n);
   const int* Ek = P.Ef + k;
 
   double carry = 0.0;   // tau1 entering the segment

@@ -54,6 +54,7 @@ struct Lane {
   bool own_stream = false;
   hipStream_t side = nullptr;     // lookahead stream (context option)
   int side_blocks = 0;            // grid cap of the panel passes on `side`
+  bool fuse = false;              // with `side`: one k_step launch per step instead
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
   std::vector<hipEvent_t> ev_p, ev_b;
   bool in_arena = false;          // d_probs, d_items carved from a caller's buffer
@@ -206,6 +207,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single, D
   if (la > 0) {   // 0: off (asked for, or the automatic choice for packed sweeps)
     HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
     L->side_blocks = la * std::max(ctx->cus, 1);
+    L->fuse = L->prec == HSRQ_PREC_FP64 && step_fusable(L->pack, L->b);
   }
   HIP_TRY(hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming));
@@ -354,6 +356,22 @@ int lane_enqueue(Lane* L, Prof* prof) {
     for (size_t i = 0; i < L->probs.size(); ++i)
       if (L->probs[i].m > 0)
         HIP_TRY(launch_equilibrate(L->d_probs + i, L->probs[i].n, L->probs[i].nrb, st));
+  if (L->fuse && !prof) {
+    // Launch s: the sweeps of step s beside the panel pass of step s - 1, on
+    // the lane's stream.  The sweep of step s starts from its tile's rows as
+    // the pass of step s - 2 left them and overwrites the coefficient
+    // buffers that pass read; the pass of step s - 1 reads what the sweep of
+    // step s - 1 wrote: both are in launch s - 1.  (The profiled form below
+    // keeps the two kernels apart, to time them.)
+    for (int s = 0; s <= L->steps; ++s) {
+      const bool sw = s < L->steps, pa = s >= 1 && L->n_gb[s - 1] > 0;
+      if (!sw && !pa) continue;
+      HIP_TRY(launch_step(L->b, L->d_probs, sw ? L->k1[s] : nullptr, sw ? L->n_k1[s] : 0,
+                          pa ? L->gb[s - 1] : nullptr, pa ? L->n_gb[s - 1] : 0, L->side_blocks,
+                          st));
+    }
+    return run(4, st, [&](hipStream_t s) { return launch_backtransform(L->d_probs, L->bt, L->n_bt, s); });
+  }
   if (L->side) {
     HIP_TRY(hipEventRecord(L->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(sd, L->ev_fork, 0));

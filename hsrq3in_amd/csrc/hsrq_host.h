@@ -171,5 +171,11 @@ hipError_t launch_rowsums(int n, const double* H, int64_t ldH, double* work, dou
                           hipStream_t st);
 // Diagonal sweep of tile size b in the given layout.
 hipError_t launch_sweep(int pack, int b, const ProbDev*, const K1Item*, int, hipStream_t);
+// Whether launch_step serves plans of this layout and tile size.
+bool step_fusable(int pack, int b);
+// One launch: the one-shift sweeps k1 of a step beside the panel pass gb of
+// the step before (capped at max_blocks blocks that loop over its items).
+hipError_t launch_step(int b, const ProbDev*, const K1Item* k1, int nk1, const GemmItem* gb,
+                       int ngb, int max_blocks, hipStream_t);
 
 }  // namespace hsrq

@@ -432,15 +432,22 @@ __device__ __forceinline__ double pick(const double (&a)[RPL], int q) {
 // beside a sweep workgroup on every CU (n = 8000, m = 800: 6.56 -> 6.31 ms
 // against 64-column chunks, up to 193 VGPRs and two panel-pass workgroups).
 template <int RPL>
-__global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* __restrict__ probs,
-                                                              const K1Item* __restrict__ items) {
+constexpr int diag_chunk_cols() { return RPL >= 2 ? 16 : 64; }   // tile columns per LDS chunk
+// LDS of the one-shift sweep, in doubles: the H chunk (column-major, ld
+// BMAX) and the [W | Z] chunk of the workgroup's 4 shifts
+template <int RPL>
+constexpr int diag_smem_doubles() { return diag_chunk_cols<RPL>() * (64 * RPL + 2 * 4); }
+
+template <int RPL>
+__device__ __forceinline__ void diag_body(const ProbDev* __restrict__ probs,
+                                          const K1Item* __restrict__ items, int bid, double* smem) {
   __builtin_amdgcn_s_setprio(2);   // latency-bound: ahead of bulk panel passes
   constexpr int BMAX = 64 * RPL;
-  constexpr int CW = (RPL >= 2) ? 16 : 64;   // tile columns per LDS chunk
-  __shared__ double Hs[CW * BMAX];           // chunk, column-major, ld BMAX
-  __shared__ double Bsh[CW * 2 * 4];         // [W | Z] chunk of the 4 shifts
+  constexpr int CW = diag_chunk_cols<RPL>();
+  double* Hs = smem;               // chunk, column-major, ld BMAX
+  double* Bsh = smem + CW * BMAX;  // [W | Z] chunk of the 4 shifts
 
-  const K1Item it = items[blockIdx.x];
+  const K1Item it = items[bid];
   const ProbDev& P = probs[it.prob];
   const int J = it.J;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -918,6 +925,13 @@ __global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* _
       gm(P.aux[J & 1])[4 * k + 3] = (double)p;
     }
   }
+}
+
+template <int RPL>
+__global__ __launch_bounds__(256, RPL == 2 ? 4 : 1) void k_diag(const ProbDev* __restrict__ probs,
+                                                              const K1Item* __restrict__ items) {
+  __shared__ double smem[diag_smem_doubles<RPL>()];
+  diag_body<RPL>(probs, items, blockIdx.x, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1447,23 +1461,35 @@ constexpr int CT = GN / 32;     // 16-column MFMA tiles per wave
 // 32 rows x 32 columns each (2 x 2 MFMA tiles).  At most 128 VGPRs: four
 // blocks per CU by registers and LDS (the 256-problem batch: 22.0 -> 21.4
 // ms), three beside a sweep workgroup.
+// LDS of the panel pass, in doubles: the H tiles As[2][KC * LDA64] (As[buf][k
+// * LDA64 + row]), the [W | Z] tiles Bs[2][GN * LDB64] (Bs[buf][col * LDB64 +
+// k]), the epilogue parameters of the block's shifts (5 x GN / 2 doubles,
+// then 2 x GN / 2 ints)
+constexpr int kUpdSmemDoubles = 2 * KC * LDA64 + 2 * GN * LDB64 + 5 * (GN / 2) + GN / 2;
+
+// Blocks bid0, bid0 + nblk, ... of the item list (LOOP: grid-stride, so that
+// a launch beside a diagonal sweep can be capped to leave room for it on
+// every CU; otherwise one block per item).
 template <bool LOOP>
-__global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ probs,
-                                                const GemmItem* __restrict__ items, int count) {
-  __shared__ double As[2][KC * LDA64];   // As[buf][k * LDA64 + row]
-  __shared__ double Bs[2][GN * LDB64];   // Bs[buf][col * LDB64 + k]
+__device__ __forceinline__ void update_body(const ProbDev* __restrict__ probs,
+                                            const GemmItem* __restrict__ items, int count,
+                                            int bid0, int nblk, double* smem) {
+  auto As = reinterpret_cast<double(*)[KC * LDA64]>(smem);
+  auto Bs = reinterpret_cast<double(*)[GN * LDB64]>(smem + 2 * KC * LDA64);
   // epilogue parameters of the block's shifts (loaded during the K loop)
-  __shared__ double pmu[GN / 2], pP[GN / 2], pZl[GN / 2], pb0w[GN / 2], pb0z[GN / 2];
-  __shared__ int psy[GN / 2], psz[GN / 2];
+  double* pmu = smem + 2 * KC * LDA64 + 2 * GN * LDB64;
+  double* pP = pmu + GN / 2;
+  double* pZl = pP + GN / 2;
+  double* pb0w = pZl + GN / 2;
+  double* pb0z = pb0w + GN / 2;
+  int* psy = reinterpret_cast<int*>(pb0z + GN / 2);
+  int* psz = psy + GN / 2;
   // per-column maxima of the epilogue, in the H tile's LDS once the K loop
   // is over (4 blocks of 256 threads fit a CU's LDS)
-  double(*red)[GN] = reinterpret_cast<double(*)[GN]>(&As[0][0]);
+  double(*red)[GN] = reinterpret_cast<double(*)[GN]>(smem);
 
-  // LOOP: grid-stride over the items, so that a launch beside a diagonal
-  // sweep can be capped to leave room for it on every CU; otherwise one
-  // block per item.
 #pragma nounroll
-  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+  for (int item = bid0; item < count; item += nblk) {
     const GemmItem it = items[item];
     const ProbDev& P = probs[it.prob];
     const int J = it.J;
@@ -1649,6 +1675,34 @@ __global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ p
     }
     if (!LOOP) break;
     __syncthreads();   // LDS is reused by the block's next item
+  }
+}
+
+template <bool LOOP>
+__global__ __launch_bounds__(256, 4) void k_update(const ProbDev* __restrict__ probs,
+                                                const GemmItem* __restrict__ items, int count) {
+  __shared__ double smem[kUpdSmemDoubles];
+  update_body<LOOP>(probs, items, count, blockIdx.x, gridDim.x, smem);
+}
+
+// One step of a plan whose panel passes overlap the sweeps: blocks [0, nk1)
+// run the one-shift sweeps of the step, blocks from nk1pad on (nk1 rounded
+// up to the 8 XCDs, so that the pass keeps its XCD-aware block order) the
+// panel pass of the step before, capped and looping over its items.  The
+// sweep of step s needs the sweep and the panel pass of step s - 1's
+// launch, the pass of step s - 1 the same: one stream, no cross-stream
+// waits (a dependent launch on one stream starts about 3.5 us after the
+// one before, one that also waits for another stream about 8.5 us).
+template <int RPL>
+__global__ __launch_bounds__(256, 4) void k_step(const ProbDev* __restrict__ probs,
+                                              const K1Item* __restrict__ k1, int nk1, int nk1pad,
+                                              const GemmItem* __restrict__ gb, int ngb) {
+  constexpr int SD = diag_smem_doubles<RPL>();
+  __shared__ double smem[SD > kUpdSmemDoubles ? SD : kUpdSmemDoubles];
+  if ((int)blockIdx.x < nk1pad) {
+    if ((int)blockIdx.x < nk1) diag_body<RPL>(probs, k1, blockIdx.x, smem);
+  } else {
+    update_body<true>(probs, gb, ngb, blockIdx.x - nk1pad, gridDim.x - nk1pad, smem);
   }
 }
 
@@ -2176,6 +2230,23 @@ hipError_t launch_diag16(int rpl, const ProbDev* d_probs, const K1Item* items, i
     case 16: hipLaunchKernelGGL(k_diag16<16>, dim3(count), dim3(256), 0, st, d_probs, items); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+bool step_fusable(int pack, int b) { return pack == 1 && b <= 128; }
+
+hipError_t launch_step(int b, const ProbDev* d_probs, const K1Item* k1, int nk1,
+                       const GemmItem* gb, int ngb, int max_blocks, hipStream_t st) {
+  const int nk1pad = (nk1 + 7) / 8 * 8;
+  const int npass = ngb > 0 ? (max_blocks > 0 && max_blocks < ngb ? max_blocks : ngb) : 0;
+  if (nk1pad + npass == 0) return hipSuccess;
+  const dim3 grid(nk1pad + npass);
+  if (b <= 64)
+    hipLaunchKernelGGL(k_step<1>, grid, dim3(256), 0, st, d_probs, k1, nk1, nk1pad, gb, ngb);
+  else if (b <= 128)
+    hipLaunchKernelGGL(k_step<2>, grid, dim3(256), 0, st, d_probs, k1, nk1, nk1pad, gb, ngb);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

@@ -212,6 +212,31 @@ def test_lookahead_batch_in_one_lane():
             assert np.array_equal(xa, xb)
 
 
+@pytest.mark.parametrize("tile", [64, 128])
+def test_fused_steps_match_two_streams(tile):
+    """With the lookahead a plan runs each step as one launch (the sweeps of
+    the step beside the panel pass of the step before, k_step); the profiled
+    execution runs the same two kernels apart, the pass on a second stream.
+    Both orders give the same vectors and status bit for bit."""
+    from hsrq3in_amd import Context, colmajor_to_device
+    H, wr = _many_shift_case()
+    c = Context(0, packing=1)
+    c.set_lookahead(3)
+    vr = torch.zeros((len(wr), H.shape[0]), dtype=torch.float64, device="cuda")
+    p = c.plan([(colmajor_to_device(H), torch.from_numpy(wr.copy()).cuda(), vr)], tile)
+    p.execute(graph=True)
+    p.sync()
+    x1, st1 = vr.cpu().numpy(), p.status()
+    vr.zero_()
+    prof = p.execute_profiled()
+    x2, st2 = vr.cpu().numpy(), p.status()
+    p.close()
+    c.close()
+    assert prof["update"][1] > 0 and prof["diag"][1] > 0
+    assert np.array_equal(x1, x2) and np.array_equal(st1, st2)
+    assert (st1 & HSRQ_CONVERGED).all()
+
+
 def test_autotune_choice_is_consistent():
     """The autotuner picks a supported configuration and a plan built after
     it gives the same vectors as an explicit request of that configuration."""

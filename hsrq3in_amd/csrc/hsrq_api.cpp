@@ -55,6 +55,7 @@ struct Lane {
   hipStream_t side = nullptr;     // lookahead stream (context option)
   int side_blocks = 0;            // grid cap of the panel passes on `side`
   bool fuse = false;              // with `side`: one k_step launch per step instead
+  int cus = 1;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
   std::vector<hipEvent_t> ev_p, ev_b;
   bool in_arena = false;          // d_probs, d_items carved from a caller's buffer
@@ -208,6 +209,7 @@ int lane_build(hsrq_context* ctx, Lane* L, int total_lane_shifts, bool single, D
     HIP_TRY(hipStreamCreateWithFlags(&L->side, hipStreamNonBlocking));
     L->side_blocks = la * std::max(ctx->cus, 1);
     L->fuse = L->prec == HSRQ_PREC_FP64 && step_fusable(L->pack, L->b);
+    L->cus = std::max(ctx->cus, 1);
   }
   HIP_TRY(hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming));
@@ -366,9 +368,11 @@ int lane_enqueue(Lane* L, Prof* prof) {
     for (int s = 0; s <= L->steps; ++s) {
       const bool sw = s < L->steps, pa = s >= 1 && L->n_gb[s - 1] > 0;
       if (!sw && !pa) continue;
-      HIP_TRY(launch_step(L->b, L->d_probs, sw ? L->k1[s] : nullptr, sw ? L->n_k1[s] : 0,
-                          pa ? L->gb[s - 1] : nullptr, pa ? L->n_gb[s - 1] : 0, L->side_blocks,
-                          st));
+      // the pass takes every workgroup slot the sweeps leave (4 per CU)
+      const int nk1 = sw ? L->n_k1[s] : 0;
+      const int cap = std::max(L->side_blocks, 4 * L->cus - (nk1 + 7) / 8 * 8);
+      HIP_TRY(launch_step(L->b, L->d_probs, sw ? L->k1[s] : nullptr, nk1,
+                          pa ? L->gb[s - 1] : nullptr, pa ? L->n_gb[s - 1] : 0, cap, st));
     }
     return run(4, st, [&](hipStream_t s) { return launch_backtransform(L->d_probs, L->bt, L->n_bt, s); });
   }

@@ -575,7 +575,10 @@ def main():
             "traffic": traffic[0] if traffic else None,
             "traffic_source": traffic[1] if traffic else None,
             "kernel": "k_update (FP64 MFMA panel pass)",
-            "launches": upd_n, "avg_launch_us": round(1e3 * upd_ms / max(upd_n, 1), 2)}
+            "launches": upd_n, "avg_launch_us": round(1e3 * upd_ms / max(upd_n, 1), 2),
+            "timed_in": "profiled solve: the panel passes as their own kernels on a second stream "
+                        "beside the sweeps (the timed solves run both in one k_step launch per "
+                        "step, which events cannot split)"}
     kernels = {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}
     # the sweeps against the HBM roofline (they are bound by the latency of
     # their rotation chain; this shows how far from bandwidth they run)

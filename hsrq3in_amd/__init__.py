@@ -27,8 +27,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# HSRQ_LIB: path of another build of the library (the in-tree one by default)
-LIB_PATH = os.environ.get("HSRQ_LIB") or os.path.join(HERE, "libhsrq.so")
+LIB_PATH = os.path.join(HERE, "libhsrq.so")
 
 HSRQ_CONVERGED = 1
 HSRQ_PERTURBED = 2

@@ -580,6 +580,20 @@ def main():
                         "beside the sweeps (the timed solves run both in one k_step launch per "
                         "step, which events cannot split)"}
     kernels = {k: {"ms": round(v[0], 4), "launches": v[1]} for k, v in prof.items()}
+    # the same pass with nothing beside it: the plan without the lookahead
+    # (sweep and pass one after the other), profiled the same way
+    c0 = Context(0, packing=args.packing)
+    c0.set_lookahead(0)
+    p0 = c0.plan([(Ht, wrt, vrt)], args.tile)
+    p0.execute()
+    p0.sync()
+    u0_ms, u0_n = p0.execute_profiled()["update"]
+    p0.close()
+    c0.close()
+    a0 = flops / (u0_ms / 1e3) / 1e12 if u0_ms > 0 else 0.0
+    roof["alone"] = {"achieved": round(a0, 3), "frac": round(a0 / PEAK_FP64_TFLOPS, 4),
+                     "avg_launch_us": round(1e3 * u0_ms / max(u0_n, 1), 2),
+                     "schedule": "no lookahead: each panel pass alone on the GPU between sweeps"}
     # the sweeps against the HBM roofline (they are bound by the latency of
     # their rotation chain; this shows how far from bandwidth they run)
     dg_ms, dg_n = prof["diag"]

@@ -340,14 +340,16 @@ def plain_variant(ctx, kind, steps, tile):
     return out
 
 
-def precision_study(graded=True, n=20000, seed=4, kmax=400, frac=0.1):
+def precision_study(graded=True, n=20000, seed=4, kmax=400, frac=0.1, tile=0):
     """Reduced-precision update path on the n = 20000, m = 2000 configuration:
     N(0, 1) upper Hessenberg (seed 4), its first 2000 real eigenvalues
     perturbed by relative 1e-12 as shifts (committed fixture), optionally 10 %
     of the indices graded by 2^k, k in [-400, 400].  FP64; FP32 panel pass
     alone; FP32 panel pass with the FP64-residual check and FP64 re-solve of
     the vectors that fail it.  Errors against the FP64 path (the test suite
-    compares the FP32 path with the FP64 oracle on the same problem)."""
+    compares the FP32 path with the FP64 oracle on the same problem).  tile 0:
+    the library's automatic tile (256-row tiles for one problem whose shifts
+    outnumber 4 x the CUs)."""
     import torch
     from hsrq3in_amd import (Context, HSRQ_CONVERGED, PREC_FP32, PREC_FP32_RAW,
                              colmajor_to_device, workloads)
@@ -365,11 +367,11 @@ def precision_study(graded=True, n=20000, seed=4, kmax=400, frac=0.1):
         ctx = Context(0, precision=prec)
         wrt = torch.from_numpy(shifts.copy()).cuda()
         vrt = torch.zeros((m, n), dtype=torch.float64, device="cuda")
-        ctx.solve(Ht, wrt, vrt, 128)                 # warm-up
+        ctx.solve(Ht, wrt, vrt, tile)                 # warm-up
         wrt.copy_(torch.from_numpy(shifts))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = ctx.solve(Ht, wrt, vrt, 128)
+        st = ctx.solve(Ht, wrt, vrt, tile)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         res = ctx.residuals(Ht, wrt, vrt)
@@ -394,7 +396,7 @@ def precision_study(graded=True, n=20000, seed=4, kmax=400, frac=0.1):
     return {"workload": f"n={n}, m={m}, N(0,1) upper Hessenberg seed {seed}, shifts = real "
                         f"eigenvalues (1+1e-12)" + (f", {int(frac * 100)} % of indices graded 2^k, "
                                                      f"k in [-{kmax}, {kmax}]" if graded else ""),
-            "modes": out}
+            "tile": tile if tile else "automatic", "modes": out}
 
 
 def served(ctx, qps_list, nreq, pool_size, seed=9):

@@ -75,8 +75,9 @@ int lookahead_blocks_per_cu(int pack, int b, int shifts, int cus);
 struct Piece {
   int prob, k0, m;
 };
-// Tile of the automatic policy for a problem of order n among `count`.
-int auto_tile(int n, int count);
+// Tile of the automatic policy for a problem of order n with `shifts`
+// shifts among `count`, on a device of `cus` compute units.
+int auto_tile(int n, long long shifts, int count, int cus);
 // Lanes of a plan: groups of pieces and the tile of each group.
 void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, int tlsz,
                   std::vector<std::vector<Piece>>& groups, std::vector<int>& tiles);

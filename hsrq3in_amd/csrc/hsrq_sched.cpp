@@ -162,8 +162,21 @@ void prob_describe(ProbDev& P, const hsrq_problem& q, int b, int its, int exact,
 // Tile of the automatic policy.  A single problem below n = 4096 takes
 // 64-row tiles (half the step latency of 128); in a batch 128-row tiles win:
 // 30.3 ms vs 33.2-34.4 ms for 64-row tiles on the small problems and 44-57
-// ms with 256-row tiles (MI355X, 256 problems of n in [200, 4000]).
-int auto_tile(int n, int count) { return count == 1 && n < 4096 ? 64 : 128; }
+// ms with 256-row tiles (MI355X, 256 problems of n in [200, 4000]).  A single
+// problem with more shifts than one-shift sweep workgroups fit on the CUs
+// (no lookahead with 128-row tiles: the panel passes set the pace) and
+// fewer than the packed sweeps take takes 256-row tiles: half the steps, so
+// half the panel passes' R~/y traffic and prologues (MI355X, FP64, 128 /
+// 256: n = 20000, m = 2000: 64.5 / 56.4 ms; n = 16000, m = 1600: 36.9 /
+// 33.9; n = 12000, m = 2000: 26.7 / 24.4; n = 8000, m = 2000: 13.6 / 13.1;
+// with the lookahead, m = 800: n = 8000 6.1 / 8.3, n = 20000 27.2 / 31.5).
+int auto_tile(int n, long long shifts, int count, int cus) {
+  if (count == 1 && n < 4096) return 64;
+  if (count == 1 && shifts > (long long)kShiftsPerK1 * std::max(cus, 1) &&
+      shifts < kPackedMinShifts)
+    return 256;
+  return 128;
+}
 
 
 // Lane assignment.  Problems are grouped by tile.  Few problems (< 8): each
@@ -180,7 +193,7 @@ void assign_lanes(const hsrq_context* ctx, int count, const hsrq_problem* in, in
   for (int i = 0; i < count; ++i) nmax = std::max(nmax, in[i].n);
   for (int i = 0; i < count; ++i)
     tile_of[i] = fixed ? choose_tile(ctx, tlsz, nmax)
-                       : choose_tile(ctx, auto_tile(in[i].n, count), 0);
+                       : choose_tile(ctx, auto_tile(in[i].n, in[i].m, count, ctx->cus), 0);
   std::vector<int> distinct(tile_of);
   std::sort(distinct.begin(), distinct.end());
   distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());

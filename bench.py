@@ -589,7 +589,8 @@ def main():
     p0 = c0.plan([(Ht, wrt, vrt)], args.tile)
     p0.execute()
     p0.sync()
-    u0_ms, u0_n = p0.execute_profiled()["update"]
+    prof0 = p0.execute_profiled()
+    u0_ms, u0_n = prof0["update"]
     p0.close()
     c0.close()
     a0 = flops / (u0_ms / 1e3) / 1e12 if u0_ms > 0 else 0.0
@@ -609,6 +610,15 @@ def main():
                   "algorithmic_bytes_per_launch": round(sb / max(dg_n, 1)),
                   "kernel": "k_diag (sweep with its own-rows update)", "launches": dg_n,
                   "avg_launch_us": round(1e3 * dg_ms / max(dg_n, 1), 2)}
+    # the same sweeps with no panel pass beside them (the plan without the
+    # lookahead above)
+    d0_ms, d0_n = prof0["diag"]
+    if d0_ms > 0:
+        g0 = sb / (d0_ms / 1e3) / 1e9
+        roof_sweep["alone"] = {"achieved": round(g0, 1), "frac": round(g0 / PEAK_HBM_GBS, 4),
+                               "avg_launch_us": round(1e3 * d0_ms / max(d0_n, 1), 2),
+                               "schedule": "no lookahead: each sweep alone on the GPU between "
+                                           "panel passes"}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "eigvecs/s", "n_gpus": 1,

@@ -134,3 +134,29 @@ def test_fullsize_cluster_first_pass_split(ctx):
     print(f"cluster first pass: 510/800 converged as the oracle; 1-|cos| cluster "
           f"{d[clus].max():.2e}, others {d[~clus].max():.2e}")
     assert d[~clus].max() <= 1e-10 and d[clus].max() <= 1e-8
+
+
+@pytest.mark.parametrize("tile", [0, 128], ids=["auto", "t128"])
+def test_config1_fixture_against_oracle(ctx, tile):
+    """The n = 1000, m = 100 configuration on its committed shifts (seed 1),
+    at the automatic tile (64 below n = 4096) and at 128: same order, every
+    shift converged, residual <= 10 n, the oracle's vectors within 1e-10."""
+    n = 1000
+    H = workloads.gaussian_hessenberg(n, 1)
+    wr = workloads.load_shifts("shifts_n1000_s1.npz")
+    m = wr.size
+    Ht = colmajor_to_device(H)
+    wrt = torch.from_numpy(wr.copy()).cuda()
+    vrt = torch.zeros((m, n), dtype=torch.float64, device="cuda")
+    st = ctx.solve(Ht, wrt, vrt, tile)
+    res = ctx.residuals(Ht, wrt, vrt)
+    x = np.asfortranarray(vrt.cpu().numpy().T)
+    xo, wo, sto = oracle.tiled_inverse_iteration(H, wr, tile or 64, 16)
+    assert np.isfinite(x).all()
+    assert np.array_equal(wrt.cpu().numpy(), wo)
+    assert (st & HSRQ_CONVERGED).all() and sto.nonconverged == 0
+    assert res.max() <= 10 * n
+    d = one_minus_abs_cos(x, xo)
+    print(f"n=1000 m=100 tile {tile or 'auto'}: max 1-|cos| {d.max():.2e}, "
+          f"max residual {res.max():.1f}")
+    assert d.max() <= 1e-10

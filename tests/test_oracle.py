@@ -315,3 +315,32 @@ def test_cluster_first_pass_split():
     x, wo, st, so, lsc = oracle.tiled_inverse_iteration_detail(H, shifts, 128, 16, 0, 0)
     assert int((so & 1).sum()) == 510 and st.nonconverged == 290
     assert np.isfinite(x).all()
+
+
+def test_config1_fixture_eigenvectors():
+    """The committed shifts of the n = 1000, m = 100 configuration (seed 1):
+    the oracle converges every shift with residual <= 10 n, and where the
+    nearest eigenvalue is well separated its vector is the eigenvector LAPACK's
+    dgeev gives for it (an independent computation of the same direction)."""
+    n, m = 1000, 100
+    H = w.gaussian_hessenberg(n, 1)
+    wr = w.load_shifts("shifts_n1000_s1.npz")
+    assert wr.shape == (m,)
+    x, wo, st = oracle.tiled_inverse_iteration(H, wr, 64, 16)
+    assert st.nonconverged == 0 and np.isfinite(x).all()
+    res, _ = oracle.residuals(H, wo, x)
+    assert res.max() <= 10 * n
+    lam, V = sla.eig(H)
+    hn = np.abs(H).sum(axis=1).max()
+    checked = 0
+    for k in range(m):
+        d = np.abs(lam - wo[k])
+        i = int(np.argmin(d))
+        gap = np.partition(np.abs(lam - lam[i]), 1)[1]
+        if abs(lam[i].imag) > 0 or gap < 1e-6 * hn:
+            continue
+        v = V[:, i].real
+        c = abs(x[:, k] @ v) / (np.linalg.norm(x[:, k]) * np.linalg.norm(v))
+        assert 1.0 - c <= 1e-8, (k, wo[k], 1.0 - c)
+        checked += 1
+    assert checked >= m // 2
